@@ -1,0 +1,143 @@
+/*
+ * skycube_amd.h -- C ABI of the MI355X skycube engine (libskycube_amd.so).
+ *
+ * Plain C types only: pointers, sizes and status codes.  No exception and no
+ * exit() crosses this boundary.  Every function returns SKY_OK (0) or a
+ * negative SKY_ERR_* code; sky_last_error() describes the last failure of a
+ * context.
+ *
+ * Semantics (identical to the reference's): points are FP32 rows, smaller is
+ * better in every dimension, compared with IEEE ordered compares.  q dominates
+ * p in subspace s (a bitmask over the d columns, bit j = column j) iff q_i <=
+ * p_i for every i in s and q_i < p_i for at least one i in s; points equal on
+ * all of s do not dominate each other.  q strictly dominates p iff q_i < p_i
+ * for every i in s.  The skyline sky(s) holds the points no point dominates;
+ * the extended skyline ext(s) the points no point strictly dominates.  NaN
+ * inputs are rejected with SKY_ERR_NAN.
+ *
+ * Membership words: a point's bitmap has one bit per subspace s, in word s/32
+ * at bit s%32, sky_words_per_point(d) words per point (2^d bits, at least one
+ * word).  Bit s is 1 iff the point is NOT in sky(s).  Subspaces outside the
+ * requested scope (s = 0, popcount(s) > max_d, and for d < 5 the unused bits
+ * s >= 2^d) have no computed skyline and are reported as 1.
+ *
+ * Contexts: one per (host thread, device).  A context owns every device
+ * allocation it makes, within the HBM budget given at creation; results of
+ * the last call stay resident on the device until the next call.  Calls on
+ * one context must not overlap.
+ */
+#ifndef SKYCUBE_AMD_H_
+#define SKYCUBE_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKY_OK 0
+#define SKY_ERR_ARG (-1)        /* bad argument (d outside [1,16], null, ...) */
+#define SKY_ERR_NAN (-2)        /* an input value is NaN */
+#define SKY_ERR_OOM_BUDGET (-3) /* the call would exceed the context's HBM budget */
+#define SKY_ERR_DEVICE (-4)     /* a HIP runtime call failed */
+#define SKY_ERR_CAPACITY (-5)   /* an output buffer is too small */
+#define SKY_ERR_STATE (-6)      /* no result of the requested kind is held */
+
+typedef struct sky_ctx sky_ctx;
+
+/* Creates a context on HIP device `device`; hbm_budget_bytes = 0 means the
+ * device's free memory.  Replaces the per-device SDSCGpu set-up done by
+ * initializeGpu (src/mdmc/MDMC.cu:1060-1084, src/sdsc/SDSC.cu:207-227). */
+int sky_ctx_create(int device, size_t hbm_budget_bytes, sky_ctx **out);
+
+/* Frees every device allocation of the context (the reference never frees
+ * its GPU state: src/mdmc/MDMC.cu:1087-1108). */
+void sky_ctx_destroy(sky_ctx *ctx);
+
+/* Text of the last failure on this context ("" if none).  Valid until the
+ * next call on the context. */
+const char *sky_last_error(const sky_ctx *ctx);
+
+/* Words per membership bitmap: max(1, 2^d / 32).  The reference's NodeAux
+ * bitmap has one extra, always-zero word (src/skycube/NodeAuxilliary.h:31). */
+uint32_t sky_words_per_point(uint32_t d);
+
+/* Device memory currently held by the context, in bytes. */
+size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
+
+/*
+ * MDMC: cuboid membership of every point.  Replaces MDMC<DIMS>::Execute's GPU
+ * path -- the full-space extended skyline (MDMC::gpu_extended_skyline ->
+ * run_skyalign, src/mdmc/MDMC.cu:504-522), the static-grid build
+ * (MDMC::build_tree_gpu, src/mdmc/MDMC.cu:664-750) and the per-point bitmap
+ * kernel loop (MDMC::tombstoning_hearting_gpu, src/mdmc/MDMC.cu:752-857).
+ *
+ * rows: n x d row-major FP32 on the host (borrowed, not kept).  max_d caps
+ * the popcount of the subspaces computed (the -m flag; max_d >= d means all).
+ * Only extended-skyline points can be in any cuboid skyline; they are
+ * returned in ext_pids (capacity n) in ascending order, their count in
+ * *n_ext.  dom_words (capacity dom_cap_words) receives n_ext bitmaps in the
+ * same order; if it is NULL or too small, the bitmaps stay on the device and
+ * sky_mdmc_fetch copies them later (the call then returns SKY_OK when
+ * dom_words is NULL and SKY_ERR_CAPACITY when it is too small).
+ */
+int sky_mdmc(sky_ctx *ctx, const float *rows, uint32_t n, uint32_t d,
+             uint32_t max_d, uint32_t *ext_pids, uint32_t *n_ext,
+             uint32_t *dom_words, uint64_t dom_cap_words);
+
+/* Same as sky_mdmc with rows already in device memory (a device pointer) and
+ * every result left on the device: nothing is copied to the host except the
+ * count.  The stream is the context's own. */
+int sky_mdmc_device(sky_ctx *ctx, const float *rows_dev, uint32_t n, uint32_t d,
+                    uint32_t max_d, uint32_t *n_ext);
+
+/* Copies the bitmaps (and ext pids, if ext_pids is not NULL) of the last
+ * sky_mdmc / sky_mdmc_device call to host memory. */
+int sky_mdmc_fetch(sky_ctx *ctx, uint32_t *ext_pids, uint32_t *dom_words,
+                   uint64_t dom_cap_words);
+
+/* Device pointers of the last MDMC result (ext pids, n_ext words each; the
+ * bitmaps, n_ext * sky_words_per_point(d) words).  Owned by the context. */
+int sky_mdmc_device_results(sky_ctx *ctx, const uint32_t **ext_pids_dev,
+                            const uint32_t **dom_words_dev);
+
+/*
+ * One-cuboid SkyAlign: sky(s) and ext(s) \ sky(s) over a candidate set.
+ * Replaces run_skyalign(full_dims, n, dimension_set, sky, ext, working_set,
+ * conf) (src/sdsc/skyAlign.cu:9-53, src/sdsc/skyAlign.cuh:321-322).
+ *
+ * rows: n_rows x d FP32 on the host.  subspace: non-zero mask over the d
+ * columns (the reference's dimension_set).  cand: candidate row ids (the
+ * reference's working_set), or NULL for all rows.  Only candidates are
+ * compared with each other.  sky_out / ext_out (capacity n_cand, or n_rows
+ * when cand is NULL) receive the row ids of sky(s) and of ext(s) \ sky(s) in
+ * ascending order.  Strictly dominated candidates are in neither.
+ */
+int sky_skyalign(sky_ctx *ctx, const float *rows, uint32_t n_rows, uint32_t d,
+                 uint32_t subspace, const uint32_t *cand, uint32_t n_cand,
+                 uint32_t *sky_out, uint32_t *n_sky, uint32_t *ext_out,
+                 uint32_t *n_ext);
+
+/*
+ * SDSC: the full lattice of cuboid skylines, top-down.  Replaces
+ * SDSC<DIMS>::Execute's GPU path (src/sdsc/SDSC.cu:248-335): every cuboid s
+ * with popcount(s) <= max_d (and the full space) is a SkyAlign over the
+ * smallest parent's sky+ext (src/sdsc/SDSC.cu:81-109,365-376).
+ *
+ * Results, per cuboid s in 1 .. 2^d - 1, as CSR arrays: sky_off and ext_off
+ * (2^d + 1 entries each; entry 0 and skipped cuboids are empty ranges),
+ * sky_ids / ext_ids (capacities sky_cap / ext_cap) with ascending row ids per
+ * cuboid.  If an id array is too small the call returns SKY_ERR_CAPACITY and
+ * the offsets still hold the sizes needed.
+ */
+int sky_sdsc(sky_ctx *ctx, const float *rows, uint32_t n, uint32_t d,
+             uint32_t max_d, uint64_t *sky_off, uint32_t *sky_ids,
+             uint64_t sky_cap, uint64_t *ext_off, uint32_t *ext_ids,
+             uint64_t ext_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKYCUBE_AMD_H_ */

@@ -1,0 +1,525 @@
+// skycube_amd.hip -- C ABI of the MI355X skycube engine (see
+// include/skycube_amd.h) and the host-side orchestration of its kernels.
+//
+// One context = one device + one stream + the device buffers of its calls,
+// grown on demand within an HBM budget.  Every HIP return code is checked
+// and turned into a SKY_ERR_* status with a message.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skycube_amd.h"
+#include "kernels.cuh"
+
+using sky::DimList;
+using sky::kBlock;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct sky_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  std::string err;
+  // device buffers
+  DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
+      status, flag, ext_pids, erows, dom, cub_tmp, counter, cand;
+  // last MDMC result
+  bool has_mdmc = false;
+  uint32_t res_n_ext = 0, res_d = 0, res_W = 0;
+};
+
+namespace {
+
+int fail(sky_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(c, SKY_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,                  \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                         \
+  } while (0)
+
+#define LAUNCHCHK()                                                                   \
+  do {                                                                                \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess)                                                             \
+      return fail(c, SKY_ERR_DEVICE, "kernel launch failed: %s (%s:%d)",              \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                         \
+  } while (0)
+
+#define TRY(expr)                 \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != SKY_OK) return rc_; \
+  } while (0)
+
+int ensure(sky_ctx* c, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (b.cap >= bytes) return SKY_OK;
+  size_t want = bytes + bytes / 8;  // headroom against regrowth
+  if (b.p) {
+    HIPCHK(hipFree(b.p));
+    c->in_use -= b.cap;
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  if (c->in_use + want > c->budget) {
+    want = bytes;
+    if (c->in_use + want > c->budget)
+      return fail(c, SKY_ERR_OOM_BUDGET,
+                  "device buffer of %zu bytes exceeds the HBM budget (%zu of %zu in use)",
+                  bytes, c->in_use, c->budget);
+  }
+  HIPCHK(hipMalloc(&b.p, want));
+  b.cap = want;
+  c->in_use += want;
+  return SKY_OK;
+}
+
+template <typename T>
+T* P(DevBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+#define SKY_DISPATCH_D(DV, ...)                          \
+  switch (DV) {                                          \
+    case 1: { constexpr int D = 1; __VA_ARGS__; } break;   \
+    case 2: { constexpr int D = 2; __VA_ARGS__; } break;   \
+    case 3: { constexpr int D = 3; __VA_ARGS__; } break;   \
+    case 4: { constexpr int D = 4; __VA_ARGS__; } break;   \
+    case 5: { constexpr int D = 5; __VA_ARGS__; } break;   \
+    case 6: { constexpr int D = 6; __VA_ARGS__; } break;   \
+    case 7: { constexpr int D = 7; __VA_ARGS__; } break;   \
+    case 8: { constexpr int D = 8; __VA_ARGS__; } break;   \
+    case 9: { constexpr int D = 9; __VA_ARGS__; } break;   \
+    case 10: { constexpr int D = 10; __VA_ARGS__; } break; \
+    case 11: { constexpr int D = 11; __VA_ARGS__; } break; \
+    case 12: { constexpr int D = 12; __VA_ARGS__; } break; \
+    case 13: { constexpr int D = 13; __VA_ARGS__; } break; \
+    case 14: { constexpr int D = 14; __VA_ARGS__; } break; \
+    case 15: { constexpr int D = 15; __VA_ARGS__; } break; \
+    case 16: { constexpr int D = 16; __VA_ARGS__; } break; \
+    default: break;                                      \
+  }
+
+DimList dims_of(uint32_t sub, uint32_t* ds) {
+  DimList L{};
+  uint32_t k = 0;
+  for (int j = 0; j < 16; ++j)
+    if (sub >> j & 1u) L.idx[k++] = j;
+  *ds = k;
+  return L;
+}
+
+int copy_count(sky_ctx* c, uint32_t* host) {
+  HIPCHK(hipMemcpyAsync(host, P<uint32_t>(c->counter), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SKY_OK;
+}
+
+// Indices i in [0, n) with flags[i] != 0, ascending, into out; count into
+// the context counter.
+int select_flagged(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out) {
+  hipcub::CountingInputIterator<uint32_t> it(0);
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, flags, out, P<uint32_t>(c->counter),
+                                        n, c->stream));
+  TRY(ensure(c, c->cub_tmp, bytes));
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->cub_tmp.p, bytes, it, flags, out,
+                                        P<uint32_t>(c->counter), n, c->stream));
+  return SKY_OK;
+}
+
+// The sky/ext split of a candidate set over subspace `sub` (ds columns).
+// Leaves, in score order, s1pid[0..m) and status[0..m) (0 sky, 1 ext, 2
+// strictly dominated) of the prefilter survivors in the context; points
+// removed by the prefilter are strictly dominated.  need_dom = 0 computes
+// only the strict part (status 0 or 2).
+int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, uint32_t sub,
+              const uint32_t* cand_dev, uint32_t n_cand, int need_dom, uint32_t* m_out) {
+  uint32_t ds = 0;
+  const DimList dims = dims_of(sub, &ds);
+  const uint32_t nc = n_cand;
+  *m_out = 0;
+  if (nc == 0) return SKY_OK;
+  TRY(ensure(c, c->key_a, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->key_b, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->pid_a, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->pid_b, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->srows, sizeof(float) * (size_t)nc * ds));
+  TRY(ensure(c, c->alive, nc));
+  TRY(ensure(c, c->s1idx, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->s1rows, sizeof(float) * (size_t)nc * ds));
+  TRY(ensure(c, c->s1keys, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->s1pid, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->status, nc));
+  TRY(ensure(c, c->counter, 4 * sizeof(uint32_t)));
+
+  HIPCHK(hipMemsetAsync(c->counter.p, 0, 4 * sizeof(uint32_t), c->stream));
+  uint32_t* err_dev = P<uint32_t>(c->counter) + 1;
+  hipLaunchKernelGGL(sky::k_score, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, rows_dev,
+                     n_rows, d, cand_dev, nc, dims, ds, P<uint32_t>(c->key_a),
+                     P<uint32_t>(c->pid_a), err_dev);
+  LAUNCHCHK();
+  uint32_t errv = 0;
+  HIPCHK(hipMemcpyAsync(&errv, err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (errv & 1u) return fail(c, SKY_ERR_ARG, "candidate id out of range (n_rows=%u)", n_rows);
+  if (errv & 2u) return fail(c, SKY_ERR_NAN, "input holds NaN");
+
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P<uint32_t>(c->key_a),
+                                            P<uint32_t>(c->key_b), P<uint32_t>(c->pid_a),
+                                            P<uint32_t>(c->pid_b), nc, 0, 32, c->stream));
+  TRY(ensure(c, c->cub_tmp, bytes));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, P<uint32_t>(c->key_a),
+                                            P<uint32_t>(c->key_b), P<uint32_t>(c->pid_a),
+                                            P<uint32_t>(c->pid_b), nc, 0, 32, c->stream));
+
+  const uint32_t npr = std::min<uint32_t>(nc, sky::kPruners);
+  SKY_DISPATCH_D(ds, {
+    hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream,
+                       rows_dev, d, P<uint32_t>(c->pid_b), nc, dims, P<float>(c->srows));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_prune<D>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream,
+                       P<float>(c->srows), P<uint32_t>(c->key_b), nc, npr, P<uint8_t>(c->alive));
+    LAUNCHCHK();
+  });
+  TRY(select_flagged(c, P<uint8_t>(c->alive), nc, P<uint32_t>(c->s1idx)));
+  uint32_t m = 0;
+  TRY(copy_count(c, &m));
+  if (m == 0) return fail(c, SKY_ERR_DEVICE, "prefilter removed every point");
+  SKY_DISPATCH_D(ds, {
+    hipLaunchKernelGGL(sky::k_gather_idx<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                       P<float>(c->srows), P<uint32_t>(c->key_b), P<uint32_t>(c->pid_b),
+                       P<uint32_t>(c->s1idx), m, P<float>(c->s1rows), P<uint32_t>(c->s1keys),
+                       P<uint32_t>(c->s1pid));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_self<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), m, need_dom,
+                       P<uint8_t>(c->status));
+    LAUNCHCHK();
+  });
+  *m_out = m;
+  return SKY_OK;
+}
+
+int check_ctx(sky_ctx* c) {
+  if (!c) return SKY_ERR_ARG;
+  c->err.clear();
+  HIPCHK(hipSetDevice(c->device));
+  return SKY_OK;
+}
+
+int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
+             uint32_t* n_ext) {
+  c->has_mdmc = false;
+  const uint32_t W = sky_words_per_point(d);
+  if (max_d == 0 || max_d > d) max_d = d;
+  *n_ext = 0;
+  if (n == 0) {
+    c->has_mdmc = true;
+    c->res_n_ext = 0;
+    c->res_d = d;
+    c->res_W = W;
+    return SKY_OK;
+  }
+  uint32_t m = 0;
+  TRY(run_split(c, rows_dev, n, d, (1u << d) - 1u, nullptr, n, 0, &m));
+  TRY(ensure(c, c->flag, n));
+  TRY(ensure(c, c->ext_pids, sizeof(uint32_t) * n));
+  HIPCHK(hipMemsetAsync(c->flag.p, 0, n, c->stream));
+  hipLaunchKernelGGL(sky::k_mark_ext, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                     P<uint8_t>(c->status), P<uint32_t>(c->s1pid), m, P<uint8_t>(c->flag));
+  LAUNCHCHK();
+  TRY(select_flagged(c, P<uint8_t>(c->flag), n, P<uint32_t>(c->ext_pids)));
+  uint32_t ne = 0;
+  TRY(copy_count(c, &ne));
+  TRY(ensure(c, c->erows, sizeof(float) * (size_t)ne * d));
+  TRY(ensure(c, c->dom, sizeof(uint32_t) * (size_t)ne * W));
+  uint32_t ds = 0;
+  const DimList dims = dims_of((1u << d) - 1u, &ds);
+  SKY_DISPATCH_D(d, {
+    hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                       rows_dev, d, P<uint32_t>(c->ext_pids), ne, dims, P<float>(c->erows));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_member<D>, dim3(ne), dim3(kBlock), 0, c->stream,
+                       P<float>(c->erows), ne, (int)max_d, P<uint32_t>(c->dom));
+    LAUNCHCHK();
+  });
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->has_mdmc = true;
+  c->res_n_ext = ne;
+  c->res_d = d;
+  c->res_W = W;
+  *n_ext = ne;
+  return SKY_OK;
+}
+
+int upload_rows(sky_ctx* c, const float* rows, uint32_t n, uint32_t d) {
+  TRY(ensure(c, c->rows, sizeof(float) * (size_t)n * d));
+  if (n)
+    HIPCHK(hipMemcpyAsync(c->rows.p, rows, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice,
+                          c->stream));
+  return SKY_OK;
+}
+
+int check_dims(sky_ctx* c, uint32_t d) {
+  if (d < 1 || d > 16) return fail(c, SKY_ERR_ARG, "d=%u outside [1,16]", d);
+  return SKY_OK;
+}
+
+// sky(s) / ext(s) of a candidate set already on the device, as ascending
+// row-id lists on the host.
+int skyalign_dev(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, uint32_t sub,
+                 const uint32_t* cand_dev, uint32_t n_cand, std::vector<uint32_t>& sky,
+                 std::vector<uint32_t>& ext) {
+  sky.clear();
+  ext.clear();
+  uint32_t m = 0;
+  TRY(run_split(c, rows_dev, n_rows, d, sub, cand_dev, n_cand, 1, &m));
+  if (m == 0) return SKY_OK;
+  std::vector<uint8_t> st(m);
+  std::vector<uint32_t> pid(m);
+  HIPCHK(hipMemcpyAsync(st.data(), c->status.p, m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(pid.data(), c->s1pid.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t j = 0; j < m; ++j) {
+    if (st[j] == 0) sky.push_back(pid[j]);
+    else if (st[j] == 1) ext.push_back(pid[j]);
+  }
+  std::sort(sky.begin(), sky.end());
+  std::sort(ext.begin(), ext.end());
+  return SKY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t sky_words_per_point(uint32_t d) { return d >= 5 ? (1u << (d - 5)) : 1u; }
+
+int sky_ctx_create(int device, size_t hbm_budget_bytes, sky_ctx** out) {
+  if (!out) return SKY_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return SKY_ERR_DEVICE;
+  sky_ctx* c = new sky_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SKY_ERR_DEVICE;
+  }
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : free_b;
+  *out = c;
+  return SKY_OK;
+}
+
+void sky_ctx_destroy(sky_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  DevBuf* bufs[] = {&c->rows,   &c->key_a,  &c->key_b,    &c->pid_a,  &c->pid_b,
+                    &c->srows,  &c->alive,  &c->s1idx,    &c->s1rows, &c->s1keys,
+                    &c->s1pid,  &c->status, &c->flag,     &c->ext_pids, &c->erows,
+                    &c->dom,    &c->cub_tmp, &c->counter, &c->cand};
+  for (DevBuf* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* sky_last_error(const sky_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t sky_ctx_bytes_in_use(const sky_ctx* c) { return c ? c->in_use : 0; }
+
+int sky_mdmc_device(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
+                    uint32_t* n_ext) {
+  TRY(check_ctx(c));
+  TRY(check_dims(c, d));
+  if (!n_ext || (n && !rows_dev)) return fail(c, SKY_ERR_ARG, "null argument");
+  return run_mdmc(c, rows_dev, n, d, max_d, n_ext);
+}
+
+int sky_mdmc(sky_ctx* c, const float* rows, uint32_t n, uint32_t d, uint32_t max_d,
+             uint32_t* ext_pids, uint32_t* n_ext, uint32_t* dom_words, uint64_t dom_cap_words) {
+  TRY(check_ctx(c));
+  TRY(check_dims(c, d));
+  if (!n_ext || !ext_pids || (n && !rows)) return fail(c, SKY_ERR_ARG, "null argument");
+  TRY(upload_rows(c, rows, n, d));
+  TRY(run_mdmc(c, P<float>(c->rows), n, d, max_d, n_ext));
+  const uint32_t ne = *n_ext;
+  if (ne)
+    HIPCHK(hipMemcpyAsync(ext_pids, c->ext_pids.p, sizeof(uint32_t) * ne, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!dom_words) return SKY_OK;
+  return sky_mdmc_fetch(c, nullptr, dom_words, dom_cap_words);
+}
+
+int sky_mdmc_fetch(sky_ctx* c, uint32_t* ext_pids, uint32_t* dom_words, uint64_t dom_cap_words) {
+  TRY(check_ctx(c));
+  if (!c->has_mdmc) return fail(c, SKY_ERR_STATE, "no MDMC result is held");
+  const uint64_t need = (uint64_t)c->res_n_ext * c->res_W;
+  if (dom_words && dom_cap_words < need)
+    return fail(c, SKY_ERR_CAPACITY, "dom_words holds %llu words, %llu needed",
+                (unsigned long long)dom_cap_words, (unsigned long long)need);
+  if (ext_pids && c->res_n_ext)
+    HIPCHK(hipMemcpyAsync(ext_pids, c->ext_pids.p, sizeof(uint32_t) * c->res_n_ext,
+                          hipMemcpyDeviceToHost, c->stream));
+  if (dom_words && need)
+    HIPCHK(hipMemcpyAsync(dom_words, c->dom.p, sizeof(uint32_t) * need, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SKY_OK;
+}
+
+int sky_mdmc_device_results(sky_ctx* c, const uint32_t** ext_pids_dev,
+                            const uint32_t** dom_words_dev) {
+  TRY(check_ctx(c));
+  if (!c->has_mdmc) return fail(c, SKY_ERR_STATE, "no MDMC result is held");
+  if (ext_pids_dev) *ext_pids_dev = P<uint32_t>(c->ext_pids);
+  if (dom_words_dev) *dom_words_dev = P<uint32_t>(c->dom);
+  return SKY_OK;
+}
+
+int sky_skyalign(sky_ctx* c, const float* rows, uint32_t n_rows, uint32_t d, uint32_t subspace,
+                 const uint32_t* cand, uint32_t n_cand, uint32_t* sky_out, uint32_t* n_sky,
+                 uint32_t* ext_out, uint32_t* n_ext) {
+  TRY(check_ctx(c));
+  TRY(check_dims(c, d));
+  if (subspace == 0 || (subspace >> d) != 0)
+    return fail(c, SKY_ERR_ARG, "subspace 0x%x invalid for d=%u", subspace, d);
+  if (!sky_out || !n_sky || !ext_out || !n_ext || (n_rows && !rows))
+    return fail(c, SKY_ERR_ARG, "null argument");
+  c->has_mdmc = false;
+  TRY(upload_rows(c, rows, n_rows, d));
+  const uint32_t nc = cand ? n_cand : n_rows;
+  const uint32_t* cand_dev = nullptr;
+  if (cand) {
+    TRY(ensure(c, c->cand, sizeof(uint32_t) * (size_t)n_cand));
+    if (n_cand)
+      HIPCHK(hipMemcpyAsync(c->cand.p, cand, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice,
+                            c->stream));
+    cand_dev = P<uint32_t>(c->cand);
+  }
+  std::vector<uint32_t> sky, ext;
+  TRY(skyalign_dev(c, P<float>(c->rows), n_rows, d, subspace, cand_dev, nc, sky, ext));
+  std::copy(sky.begin(), sky.end(), sky_out);
+  std::copy(ext.begin(), ext.end(), ext_out);
+  *n_sky = (uint32_t)sky.size();
+  *n_ext = (uint32_t)ext.size();
+  return SKY_OK;
+}
+
+int sky_sdsc(sky_ctx* c, const float* rows, uint32_t n, uint32_t d, uint32_t max_d,
+             uint64_t* sky_off, uint32_t* sky_ids, uint64_t sky_cap, uint64_t* ext_off,
+             uint32_t* ext_ids, uint64_t ext_cap) {
+  TRY(check_ctx(c));
+  TRY(check_dims(c, d));
+  if (!sky_off || !ext_off || (n && !rows)) return fail(c, SKY_ERR_ARG, "null argument");
+  if (max_d == 0 || max_d > d) max_d = d;
+  c->has_mdmc = false;
+  const uint32_t S = 1u << d, full = S - 1u;
+  std::vector<std::vector<uint32_t>> lsky(S), lext(S);
+  TRY(upload_rows(c, rows, n, d));
+  const float* rows_dev = P<float>(c->rows);
+  // full space over every point (SDSC.cu:274)
+  TRY(skyalign_dev(c, rows_dev, n, d, full, nullptr, n, lsky[full], lext[full]));
+  std::vector<uint32_t> cand;
+  // levels top-down, skipping popcount > max_d (SDSC.cu:291-294)
+  for (int lev = (int)d - 1; lev >= 1; --lev) {
+    if ((uint32_t)lev > max_d) continue;
+    for (uint32_t s = 1; s < full; ++s) {
+      if (__builtin_popcount(s) != lev) continue;
+      // parents in generate_lattice order (descending id); the smallest
+      // sky+ext wins, the full space stands in when parents were skipped
+      // (src/sdsc/SDSC.cu:81-104,365-376)
+      uint32_t src = full;
+      uint32_t first = 0;
+      for (int j = (int)d - 1; j >= 0; --j)
+        if (!(s >> j & 1u)) {
+          first = s | (1u << j);
+          break;
+        }
+      if (!lsky[first].empty()) {
+        size_t best = (size_t)-1;
+        for (int j = (int)d - 1; j >= 0; --j) {
+          if (s >> j & 1u) continue;
+          const uint32_t par = s | (1u << j);
+          const size_t sz = lsky[par].size() + lext[par].size();
+          if (sz < best) {
+            best = sz;
+            src = par;
+          }
+        }
+      }
+      cand.assign(lsky[src].begin(), lsky[src].end());
+      cand.insert(cand.end(), lext[src].begin(), lext[src].end());
+      TRY(ensure(c, c->cand, sizeof(uint32_t) * std::max<size_t>(cand.size(), 1)));
+      if (!cand.empty())
+        HIPCHK(hipMemcpyAsync(c->cand.p, cand.data(), sizeof(uint32_t) * cand.size(),
+                              hipMemcpyHostToDevice, c->stream));
+      TRY(skyalign_dev(c, rows_dev, n, d, s, P<uint32_t>(c->cand), (uint32_t)cand.size(),
+                       lsky[s], lext[s]));
+    }
+  }
+  uint64_t ts = 0, te = 0;
+  sky_off[0] = 0;
+  ext_off[0] = 0;
+  for (uint32_t s = 1; s < S; ++s) {
+    ts += lsky[s].size();
+    te += lext[s].size();
+  }
+  sky_off[0] = ext_off[0] = 0;
+  uint64_t os = 0, oe = 0;
+  for (uint32_t s = 0; s < S; ++s) {
+    sky_off[s] = os;
+    ext_off[s] = oe;
+    if (s == 0) continue;
+    if (sky_ids && os + lsky[s].size() <= sky_cap)
+      std::copy(lsky[s].begin(), lsky[s].end(), sky_ids + os);
+    if (ext_ids && oe + lext[s].size() <= ext_cap)
+      std::copy(lext[s].begin(), lext[s].end(), ext_ids + oe);
+    os += lsky[s].size();
+    oe += lext[s].size();
+  }
+  sky_off[S] = os;
+  ext_off[S] = oe;
+  if (!sky_ids || !ext_ids || ts > sky_cap || te > ext_cap)
+    return fail(c, SKY_ERR_CAPACITY, "sdsc needs %llu sky ids and %llu ext ids",
+                (unsigned long long)ts, (unsigned long long)te);
+  return SKY_OK;
+}
+
+}  // extern "C"

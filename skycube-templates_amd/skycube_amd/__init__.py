@@ -1,0 +1,240 @@
+"""Python host binding of the MI355X skycube engine.
+
+A thin ctypes layer over the C ABI of ``include/skycube_amd.h``
+(libskycube_amd.so, HIP kernels for gfx950).  It mirrors the reference's
+algorithm objects (src/mdmc/MDMC.h:34-67, src/sdsc/SDSC.h:18-54): ``mdmc``
+returns per-point cuboid membership bitmaps, ``skyalign`` one cuboid's
+sky/ext split, ``sdsc`` the lattice of cuboid skylines.
+
+There is no fallback: if the library is missing or the device call fails,
+the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "SkyError", "Context", "library_path", "load", "words_per_point",
+    "in_scope_mask", "members", "SKY_OK",
+]
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_PKG_ROOT, "libskycube_amd.so")
+
+SKY_OK = 0
+_ERRORS = {
+    -1: "SKY_ERR_ARG", -2: "SKY_ERR_NAN", -3: "SKY_ERR_OOM_BUDGET",
+    -4: "SKY_ERR_DEVICE", -5: "SKY_ERR_CAPACITY", -6: "SKY_ERR_STATE",
+}
+
+# Exported symbols of libskycube_amd.so (the declarations of skycube_amd.h).
+EXPORTS = (
+    "sky_ctx_create", "sky_ctx_destroy", "sky_last_error", "sky_words_per_point",
+    "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
+    "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc",
+)
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_lib = None
+
+
+class SkyError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def load():
+    """Loads libskycube_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise FileNotFoundError(
+            f"{_LIB_PATH} is missing: build it with `make -C {_PKG_ROOT}` "
+            "(or __graft_entry__.build())")
+    L = ctypes.CDLL(_LIB_PATH)
+    L.sky_ctx_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+    L.sky_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.sky_ctx_destroy.restype = None
+    L.sky_last_error.argtypes = [ctypes.c_void_p]
+    L.sky_last_error.restype = ctypes.c_char_p
+    L.sky_words_per_point.argtypes = [ctypes.c_uint32]
+    L.sky_words_per_point.restype = ctypes.c_uint32
+    L.sky_ctx_bytes_in_use.argtypes = [ctypes.c_void_p]
+    L.sky_ctx_bytes_in_use.restype = ctypes.c_size_t
+    L.sky_mdmc.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                           ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.c_uint64]
+    L.sky_mdmc_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_uint32, _u32p]
+    L.sky_mdmc_fetch.argtypes = [ctypes.c_void_p, _u32p, _u32p, ctypes.c_uint64]
+    L.sky_mdmc_device_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(ctypes.c_void_p)]
+    L.sky_skyalign.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p, _u32p, _u32p,
+                               _u32p]
+    L.sky_sdsc.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                           ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p,
+                           ctypes.c_uint64]
+    _lib = L
+    return L
+
+
+def words_per_point(d: int) -> int:
+    """max(1, 2^d / 32): words of one membership bitmap."""
+    return 1 << (d - 5) if d >= 5 else 1
+
+
+def in_scope_mask(d: int, max_d: int | None = None) -> np.ndarray:
+    """[W] uint32 words with bit s set iff 1 <= s < 2^d and popcount(s) <= max_d."""
+    max_d = d if max_d is None else max_d
+    W = words_per_point(d)
+    s = np.arange(W * 32, dtype=np.uint64)
+    pc = np.zeros_like(s)
+    t = s.copy()
+    while t.any():
+        pc += t & 1
+        t >>= 1
+    ok = (s >= 1) & (s < (1 << d)) & (pc <= max_d)
+    bits = ok.reshape(W, 32).astype(np.uint64) << np.arange(32, dtype=np.uint64)
+    return bits.sum(axis=1).astype(np.uint32)
+
+
+def members(dom: np.ndarray, ext_pids: np.ndarray, s: int) -> np.ndarray:
+    """Skyline of cuboid s from membership bitmaps: the ext pids whose bit s
+    is clear (the query of getsubspace, src/utilities/utilities.h:253-267)."""
+    w, b = divmod(int(s), 32)
+    return np.asarray(ext_pids)[((dom[:, w] >> np.uint32(b)) & 1) == 0]
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    """One device context: owns device memory within an HBM budget."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.sky_ctx_create(int(device), int(hbm_budget), ctypes.byref(h))
+        if rc != SKY_OK:
+            raise SkyError(rc, f"sky_ctx_create(device={device}) failed")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sky_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc: int):
+        if rc != SKY_OK:
+            raise SkyError(rc, self._L.sky_last_error(self._h).decode(errors="replace"))
+
+    def bytes_in_use(self) -> int:
+        return int(self._L.sky_ctx_bytes_in_use(self._h))
+
+    @staticmethod
+    def _rows(rows) -> np.ndarray:
+        x = np.ascontiguousarray(rows, dtype=np.float32)
+        if x.ndim != 2:
+            raise ValueError("rows must be an n x d array")
+        return x
+
+    def mdmc(self, rows, max_d: int | None = None):
+        """(ext_pids ascending [n_ext], membership bitmaps [n_ext, W] uint32)."""
+        x = self._rows(rows)
+        n, d = x.shape
+        W = words_per_point(d)
+        ext = np.zeros(max(n, 1), dtype=np.uint32)
+        ne = np.zeros(1, dtype=np.uint32)
+        self._check(self._L.sky_mdmc(self._h, _ptr(x, _f32p), n, d, 0 if max_d is None else max_d,
+                                     _ptr(ext, _u32p), _ptr(ne, _u32p), None, 0))
+        k = int(ne[0])
+        dom = np.zeros(max(k * W, 1), dtype=np.uint32)
+        self._check(self._L.sky_mdmc_fetch(self._h, None, _ptr(dom, _u32p), dom.size))
+        return ext[:k].copy(), dom[: k * W].reshape(k, W)
+
+    def mdmc_device(self, rows_dev_ptr: int, n: int, d: int, max_d: int | None = None) -> int:
+        """Runs on rows already in device memory; results stay on the device."""
+        ne = np.zeros(1, dtype=np.uint32)
+        self._check(self._L.sky_mdmc_device(self._h, ctypes.c_void_p(rows_dev_ptr), n, d,
+                                            0 if max_d is None else max_d, _ptr(ne, _u32p)))
+        return int(ne[0])
+
+    def mdmc_fetch(self, n_ext: int, d: int):
+        W = words_per_point(d)
+        ext = np.zeros(max(n_ext, 1), dtype=np.uint32)
+        dom = np.zeros(max(n_ext * W, 1), dtype=np.uint32)
+        self._check(self._L.sky_mdmc_fetch(self._h, _ptr(ext, _u32p), _ptr(dom, _u32p), dom.size))
+        return ext[:n_ext].copy(), dom[: n_ext * W].reshape(n_ext, W)
+
+    def mdmc_device_results(self):
+        e = ctypes.c_void_p()
+        w = ctypes.c_void_p()
+        self._check(self._L.sky_mdmc_device_results(self._h, ctypes.byref(e), ctypes.byref(w)))
+        return e.value, w.value
+
+    def skyalign(self, rows, subspace: int, cand=None):
+        """(sky ids ascending, ext-but-not-sky ids ascending) of cuboid `subspace`."""
+        x = self._rows(rows)
+        n, d = x.shape
+        if cand is None:
+            c = n
+            cp = None
+        else:
+            cand = np.ascontiguousarray(cand, dtype=np.uint32)
+            c = cand.size
+            cp = _ptr(cand, _u32p)
+        sky = np.zeros(max(c, 1), dtype=np.uint32)
+        ext = np.zeros(max(c, 1), dtype=np.uint32)
+        ns = np.zeros(1, dtype=np.uint32)
+        ne = np.zeros(1, dtype=np.uint32)
+        self._check(self._L.sky_skyalign(self._h, _ptr(x, _f32p), n, d, int(subspace), cp, c,
+                                         _ptr(sky, _u32p), _ptr(ns, _u32p), _ptr(ext, _u32p),
+                                         _ptr(ne, _u32p)))
+        return sky[: int(ns[0])].copy(), ext[: int(ne[0])].copy()
+
+    def sdsc(self, rows, max_d: int | None = None):
+        """CSR lattice: (sky_off [2^d+1], sky_ids, ext_off [2^d+1], ext_ids)."""
+        x = self._rows(rows)
+        n, d = x.shape
+        S = 1 << d
+        so = np.zeros(S + 1, dtype=np.uint64)
+        eo = np.zeros(S + 1, dtype=np.uint64)
+        rc = self._L.sky_sdsc(self._h, _ptr(x, _f32p), n, d, 0 if max_d is None else max_d,
+                              _ptr(so, _u64p), None, 0, _ptr(eo, _u64p), None, 0)
+        if rc != -5:
+            self._check(rc)
+        sky = np.zeros(max(int(so[S]), 1), dtype=np.uint32)
+        ext = np.zeros(max(int(eo[S]), 1), dtype=np.uint32)
+        self._check(self._L.sky_sdsc(self._h, _ptr(x, _f32p), n, d, 0 if max_d is None else max_d,
+                                     _ptr(so, _u64p), _ptr(sky, _u32p), sky.size, _ptr(eo, _u64p),
+                                     _ptr(ext, _u32p), ext.size))
+        return so, sky[: int(so[S])].copy(), eo, ext[: int(eo[S])].copy()

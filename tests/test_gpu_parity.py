@@ -1,0 +1,144 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI.
+
+Bit-exact: the ext pids and every membership word must be identical (bits of
+out-of-scope subspaces are 1 on both sides).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import npref
+import skycube_amd
+from oracle import skyorc
+from skycube_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def check_mdmc(ctx, x, max_d=None, tag=""):
+    n, d = x.shape
+    _say(f"mdmc parity {tag} n={n} d={d} max_d={max_d}")
+    ext, dom = ctx.mdmc(x, max_d)
+    oext, odom = skyorc.mdmc(x, d if max_d is None else max_d)
+    assert np.array_equal(ext, oext), f"ext differs: {ext.size} vs {oext.size}"
+    if not np.array_equal(dom, odom):
+        bad = np.flatnonzero((dom != odom).any(axis=1))
+        raise AssertionError(f"{bad.size} of {dom.shape[0]} bitmaps differ (first pid {ext[bad[0]]})")
+    return ext, dom
+
+
+@pytest.mark.parametrize("d", list(range(1, 17)))
+def test_mdmc_every_d_ties(ctx, d):
+    rng = np.random.default_rng(1000 + d)
+    n = 400 if d <= 12 else 150
+    check_mdmc(ctx, npref.tie_heavy(rng, n, d, 3), tag="ties")
+
+
+@pytest.mark.parametrize("d", [2, 5, 8, 11, 14, 16])
+def test_mdmc_every_d_continuous(ctx, d):
+    rng = np.random.default_rng(2000 + d)
+    check_mdmc(ctx, rng.random((700, d), dtype=np.float32), tag="uniform")
+
+
+@pytest.mark.parametrize("d,max_d", [(6, 1), (9, 3), (12, 5), (16, 4)])
+def test_mdmc_max_d(ctx, d, max_d):
+    rng = np.random.default_rng(3000 + d)
+    check_mdmc(ctx, rng.integers(0, 6, size=(500, d)).astype(np.float32), max_d, tag="cap")
+
+
+def test_mdmc_edge_cases(ctx):
+    # empty input
+    ext, dom = ctx.mdmc(np.zeros((0, 4), dtype=np.float32))
+    assert ext.size == 0 and dom.shape == (0, 1)
+    # a single point is in every cuboid skyline
+    ext, dom = ctx.mdmc(np.array([[0.5, 0.25, 1.0]], dtype=np.float32))
+    assert ext.tolist() == [0]
+    assert np.array_equal(dom[0], ~skycube_amd.in_scope_mask(3))
+    # all points equal: all are members everywhere
+    x = np.full((300, 7), 3.0, dtype=np.float32)
+    ext, dom = ctx.mdmc(x)
+    assert ext.size == 300
+    assert (dom == ~skycube_amd.in_scope_mask(7)[None, :]).all()
+    # -0.0 and +0.0 are equal values
+    x = np.array([[0.0, 1.0], [-0.0, 1.0], [1.0, 0.0]], dtype=np.float32)
+    check_mdmc(ctx, x, tag="signed zero")
+    # negative and large magnitudes
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal((500, 6)) * 1e6).astype(np.float32)
+    check_mdmc(ctx, x, tag="wide range")
+
+
+def test_mdmc_rejects_nan(ctx):
+    x = np.zeros((10, 3), dtype=np.float32)
+    x[3, 2] = np.nan
+    with pytest.raises(skycube_amd.SkyError) as e:
+        ctx.mdmc(x)
+    assert e.value.code == -2
+
+
+def test_mdmc_rejects_bad_d(ctx):
+    with pytest.raises(skycube_amd.SkyError):
+        ctx.mdmc(np.zeros((4, 17), dtype=np.float32))
+
+
+def test_mdmc_workload_independent_full(ctx):
+    """The full independent workload: n=100,000, d=8, all 255 cuboids."""
+    x = datagen.workload("independent_distinct")
+    ext, _ = check_mdmc(ctx, x, tag="independent_distinct full")
+    assert ext.size == 9984
+
+
+def test_mdmc_workload_anticorrelated_sample(ctx):
+    check_mdmc(ctx, datagen.workload("anticorrelated_plane", 20000), tag="anticorrelated 20k")
+
+
+def test_mdmc_workload_integer_ties_sample(ctx):
+    check_mdmc(ctx, datagen.workload("integer_ties", 60000), max_d=8, tag="integer_ties 60k")
+
+
+def test_mdmc_batch_mix(ctx):
+    """A few instances of the batched workload, one call each."""
+    for spec in datagen.batch_specs(24, seed=4)[:24:3]:
+        n, d, dist, s = spec
+        check_mdmc(ctx, datagen.batch_instance((min(n, 6000), d, dist, s)), tag=f"{dist}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_skyalign_matches_oracle(ctx, seed):
+    rng = np.random.default_rng(4000 + seed)
+    n, d = 3000, 7
+    x = rng.integers(0, 9, size=(n, d)).astype(np.float32)
+    for sub in (1, 0b11, 0b1010101, 0b1111111, int(rng.integers(1, 128))):
+        for cand in (None, np.sort(rng.choice(n, size=1200, replace=False)).astype(np.uint32)):
+            _say(f"skyalign seed={seed} sub={sub:#x} cand={'all' if cand is None else cand.size}")
+            sky, ext = ctx.skyalign(x, sub, cand)
+            st = skyorc.skyalign(x, sub, cand)
+            ids = np.arange(n, dtype=np.uint32) if cand is None else cand
+            assert np.array_equal(sky, np.sort(ids[st == 0]))
+            assert np.array_equal(ext, np.sort(ids[st == 1]))
+
+
+@pytest.mark.parametrize("n,d,max_d", [(300, 5, 5), (400, 6, 3), (250, 8, 8)])
+def test_sdsc_matches_stsc(ctx, n, d, max_d):
+    rng = np.random.default_rng(5000 + d)
+    x = rng.integers(0, 7, size=(n, d)).astype(np.float32)
+    _say(f"sdsc n={n} d={d} max_d={max_d}")
+    so, sky, eo, ext = ctx.sdsc(x, max_d)
+    dom, sdom, skyn, extn = skyorc.stsc(x, max_d)
+    S = 1 << d
+    D = npref.unpack_bits(dom, S)
+    SD = npref.unpack_bits(sdom, S)
+    for s in range(1, S):
+        if bin(s).count("1") > max_d and s != S - 1:
+            assert so[s + 1] == so[s] and eo[s + 1] == eo[s]
+            continue
+        got_sky = sky[so[s]:so[s + 1]]
+        got_ext = ext[eo[s]:eo[s + 1]]
+        assert np.array_equal(got_sky, np.flatnonzero(~D[:, s])), s
+        assert np.array_equal(got_ext, np.flatnonzero(D[:, s] & ~SD[:, s])), s
