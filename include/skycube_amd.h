@@ -66,6 +66,19 @@ uint32_t sky_words_per_point(uint32_t d);
 /* Device memory currently held by the context, in bytes. */
 size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
+/* Device-time breakdown of the last MDMC call, from HIP events recorded on
+ * the context's stream while timing is enabled (off by default; it adds a
+ * few event records per call).  No reference counterpart: the reference
+ * times only the whole Execute() window (src/main.h:324-336). */
+#define SKY_STAGE_SCORE_SORT 0 /* Manhattan-norm keys, radix sort, gather */
+#define SKY_STAGE_PRUNE 1      /* prefilter against the lowest-score points */
+#define SKY_STAGE_SELF 2       /* survivors against each other */
+#define SKY_STAGE_COMPACT 3    /* extended skyline in id order, row gather */
+#define SKY_STAGE_MEMBER 4     /* per-point membership bitmaps */
+#define SKY_STAGE_COUNT 5
+int sky_ctx_set_timing(sky_ctx *ctx, int enable);
+int sky_ctx_stage_times(const sky_ctx *ctx, float *ms, int count);
+
 /*
  * MDMC: cuboid membership of every point.  Replaces MDMC<DIMS>::Execute's GPU
  * path -- the full-space extended skyline (MDMC::gpu_extended_skyline ->

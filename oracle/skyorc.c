@@ -266,28 +266,123 @@ int orc_skyalign(const float *rows, uint32_t n_rows, int d, uint32_t sub,
 
 /* Extended skyline of the full space: ascending row ids not strictly
  * dominated (sky + ext of src/sdsc/skyAlign.cuh as used by MDMC,
- * src/mdmc/MDMC.cu:504-522). */
+ * src/mdmc/MDMC.cu:504-522).  Same method as orc_skyalign, with the strict
+ * test alone, which stops at the first column where q is not better. */
+static inline int strictly_dominates(const float *q, const float *p, int d) {
+  for (int i = 0; i < d; ++i)
+    if (!(q[i] < p[i]))
+      return 0;
+  return 1;
+}
+
+/* The strict-only split behind orc_ext_skyline.  With stride > 1 only every
+ * stride-th survivor of the prefilter gets the survivor pass (for timing a
+ * bounded part of a large input; dead[] is then incomplete).  Wall times of
+ * the two passes go to *t_prune / *t_self when not NULL. */
+static int ext_split(const float *rows, uint32_t n, int d, uint32_t stride,
+                     uint8_t *in, double *t_prune, double *t_self,
+                     uint32_t *n_surv) {
+  const uint32_t full = (1u << d) - 1u;
+  scored_t *sc = (scored_t *)malloc(sizeof(scored_t) * n);
+  uint8_t *dead = (uint8_t *)calloc(n, 1);
+  uint32_t *surv = (uint32_t *)malloc(sizeof(uint32_t) * n);
+  if (!sc || !dead || !surv) {
+    free(sc);
+    free(dead);
+    free(surv);
+    return ORC_ERR_OOM;
+  }
+  double t0 = omp_get_wtime();
+  for (uint32_t k = 0; k < n; ++k) {
+    sc[k].score = score_of(rows + (size_t)k * d, full, d);
+    sc[k].idx = k;
+  }
+  qsort(sc, n, sizeof(scored_t), cmp_scored);
+  const uint32_t npr = n < 1024 ? n : 1024;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (uint32_t r = 0; r < n; ++r) {
+    const float *p = rows + (size_t)sc[r].idx * d;
+    for (uint32_t t = 0; t < npr; ++t) {
+      if (sc[t].score > sc[r].score)
+        break;
+      if (t != r && strictly_dominates(rows + (size_t)sc[t].idx * d, p, d)) {
+        dead[r] = 1;
+        break;
+      }
+    }
+  }
+  uint32_t ns = 0;
+  for (uint32_t r = 0; r < n; ++r)
+    if (!dead[r])
+      surv[ns++] = r;
+  double t1 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 64)
+  for (uint32_t a = 0; a < ns; a += stride) {
+    const uint32_t r = surv[a];
+    const float *p = rows + (size_t)sc[r].idx * d;
+    for (uint32_t b = 0; b < ns; ++b) {
+      const uint32_t t = surv[b];
+      if (sc[t].score > sc[r].score)
+        break;
+      if (t != r && strictly_dominates(rows + (size_t)sc[t].idx * d, p, d)) {
+        dead[r] = 1;
+        break;
+      }
+    }
+  }
+  double t2 = omp_get_wtime();
+  if (t_prune)
+    *t_prune = t1 - t0;
+  if (t_self)
+    *t_self = t2 - t1;
+  if (n_surv)
+    *n_surv = ns;
+  if (in)
+    for (uint32_t r = 0; r < n; ++r)
+      in[sc[r].idx] = !dead[r];
+  free(sc);
+  free(dead);
+  free(surv);
+  return ORC_OK;
+}
+
 int orc_ext_skyline(const float *rows, uint32_t n, int d, uint32_t *ext_idx,
                     uint32_t *n_ext) {
+  if (d < 1 || d > 16)
+    return ORC_ERR_ARG;
   if (n == 0) {
     *n_ext = 0;
     return ORC_OK;
   }
-  uint8_t *st = (uint8_t *)malloc(n);
-  if (!st)
+  if (has_nan(rows, n, d))
+    return ORC_ERR_NAN;
+  uint8_t *in = (uint8_t *)calloc(n, 1);
+  if (!in)
     return ORC_ERR_OOM;
-  int rc = orc_skyalign(rows, n, d, (1u << d) - 1u, NULL, 0, st);
-  if (rc != ORC_OK) {
-    free(st);
-    return rc;
+  int rc;
+  if (g_mutation != 0) { /* planted mistakes live in pair_masks: use it */
+    rc = orc_skyalign(rows, n, d, (1u << d) - 1u, NULL, 0, in);
+    for (uint32_t i = 0; i < n; ++i)
+      in[i] = in[i] != 2;
+  } else {
+    rc = ext_split(rows, n, d, 1, in, NULL, NULL, NULL);
   }
   uint32_t m = 0;
-  for (uint32_t i = 0; i < n; ++i)
-    if (st[i] != 2)
+  for (uint32_t i = 0; rc == ORC_OK && i < n; ++i)
+    if (in[i])
       ext_idx[m++] = i;
   *n_ext = m;
-  free(st);
-  return ORC_OK;
+  free(in);
+  return rc;
+}
+
+/* Times the extended-skyline split with the survivor pass run on every
+ * stride-th survivor only (a bounded part of a large input). */
+int orc_ext_skyline_timed(const float *rows, uint32_t n, int d, uint32_t stride,
+                          double *t_prune, double *t_self, uint32_t *n_surv) {
+  if (d < 1 || d > 16 || stride == 0 || n == 0)
+    return ORC_ERR_ARG;
+  return ext_split(rows, n, d, stride, NULL, t_prune, t_self, n_surv);
 }
 
 /* ----------------------------------------------------------------------- */
@@ -383,6 +478,37 @@ int orc_mdmc(const float *rows, uint32_t n, int d, int max_d, uint32_t *ext_idx,
     free(planes);
   }
   return oom ? ORC_ERR_OOM : ORC_OK;
+}
+
+/* Membership bitmaps of the extended-skyline points ext_idx[sample[k]] only
+ * (k < n_sample), each against the whole extended skyline: the per-point
+ * cost of orc_mdmc on a sample, for timing the CPU path on a bounded part of
+ * a large input. */
+int orc_mdmc_sample(const float *rows, int d, int max_d, const uint32_t *ext_idx,
+                    uint32_t n_ext, const uint32_t *sample, uint32_t n_sample,
+                    uint32_t *dom_words) {
+  if (d < 1 || d > 16 || max_d < 1)
+    return ORC_ERR_ARG;
+  if (max_d > d)
+    max_d = d;
+  const int W = orc_words_per_point(d);
+  int bad = 0;
+#pragma omp parallel
+  {
+    uint32_t *planes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * d);
+#pragma omp for schedule(dynamic, 4)
+    for (uint32_t k = 0; k < n_sample; ++k) {
+      if (!planes || sample[k] >= n_ext) {
+#pragma omp atomic write
+        bad = 1;
+        continue;
+      }
+      membership_one(rows, d, ext_idx, n_ext, ext_idx[sample[k]], max_d,
+                     dom_words + (size_t)k * W, planes);
+    }
+    free(planes);
+  }
+  return bad ? ORC_ERR_ARG : ORC_OK;
 }
 
 /* ----------------------------------------------------------------------- */

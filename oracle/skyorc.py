@@ -43,6 +43,11 @@ def lib():
                                           ctypes.c_uint32]
         L.orc_score_tie_count.restype = ctypes.c_uint64
         L.orc_set_mutation.argtypes = [ctypes.c_int]
+        L.orc_ext_skyline_timed.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_int,
+                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), _u32p]
+        L.orc_mdmc_sample.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_uint32,
+                                      _u32p, ctypes.c_uint32, _u32p]
         _lib = L
     return _lib
 
@@ -147,3 +152,33 @@ def score_tie_count(x, ext, max_d: int | None = None) -> int:
     max_d = d if max_d is None else max_d
     ext = np.ascontiguousarray(ext, dtype=np.uint32)
     return int(lib().orc_score_tie_count(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p), ext.size))
+
+
+def mdmc_sample(x, ext, sample, max_d: int | None = None) -> np.ndarray:
+    """Bitmaps [len(sample), W] of the ext points ext[sample] only."""
+    x = _rows(x)
+    n, d = x.shape
+    max_d = d if max_d is None else max_d
+    W = words_per_point(d)
+    ext = np.ascontiguousarray(ext, dtype=np.uint32)
+    sample = np.ascontiguousarray(sample, dtype=np.uint32)
+    out = np.zeros(max(sample.size, 1) * W, dtype=np.uint32)
+    rc = lib().orc_mdmc_sample(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p), ext.size,
+                               _ptr(sample, _u32p), sample.size, _ptr(out, _u32p))
+    if rc != 0:
+        raise RuntimeError(f"orc_mdmc_sample rc={rc}")
+    return out[: sample.size * W].reshape(sample.size, W)
+
+
+def ext_skyline_timed(x, stride: int):
+    """(prefilter seconds, sampled survivor-pass seconds, survivors)."""
+    x = _rows(x)
+    n, d = x.shape
+    tp = ctypes.c_double()
+    ts = ctypes.c_double()
+    ns = np.zeros(1, dtype=np.uint32)
+    rc = lib().orc_ext_skyline_timed(_ptr(x, _f32p), n, d, int(stride), ctypes.byref(tp),
+                                     ctypes.byref(ts), _ptr(ns, _u32p))
+    if rc != 0:
+        raise RuntimeError(f"orc_ext_skyline_timed rc={rc}")
+    return tp.value, ts.value, int(ns[0])

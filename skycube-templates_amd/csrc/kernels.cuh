@@ -232,69 +232,176 @@ __device__ __forceinline__ uint32_t scope_mask(uint32_t k, int D, int max_d) {
   return m;
 }
 
-// Per-point cuboid membership, one workgroup per extended-skyline point p
-// (the contract of produce_bitmap_iterative_multi_partition_templated_async,
-// src/utilities/kernels.cuh:415-908, without its approximations).
-//
-// Pass 1 over every extended-skyline q builds three lattice bitmaps in LDS:
-//   A0: lt|eq of pairs without ties      -> closure D0, exact part
-//   A1: lt|eq of every dominating pair   -> closure D_over >= D
-//   TE: eq of pairs with ties            -> closure T (tie-touched subspaces)
-// For s outside T, D_over[s] is exact: a pair marking s has s not inside its
-// eq.  Inside T, s in D0 is dominated and s outside D_over is not; the rest,
-// X = T & D_over & ~D0 (usually empty), is decided exactly by a second pass
-// over the tie pairs: s is dominated iff some q has s inside lt|eq and s
-// meeting lt.  Output: bit s = 1 iff p is not in sky(s); out-of-scope bits 1.
+// Column minima of the extended skyline, as order keys (atomicMin), for the
+// boundary columns of the membership kernel.  minkey must start at ~0.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows, uint32_t m,
-                                                   int max_d, uint32_t* __restrict__ dom) {
-  constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
-  __shared__ uint32_t A0[W];
-  __shared__ uint32_t A1[W];
-  __shared__ uint32_t TE[W];
-  __shared__ uint16_t xl[kXCap];
-  __shared__ int s_any;
-  __shared__ int s_cnt;
-  const uint32_t a = blockIdx.x;
-  const int tid = threadIdx.x;
-  float p[D];
+__global__ void k_colmin(const float* __restrict__ erows, uint32_t m, uint32_t* __restrict__ minkey) {
+  uint32_t best[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) p[k] = erows[(size_t)a * D + k];
-  for (int k = tid; k < W; k += kBlock) {
-    A0[k] = 0;
-    A1[k] = 0;
-    TE[k] = 0;
+  for (int k = 0; k < D; ++k) best[k] = 0xFFFFFFFFu;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) best[k] = min(best[k], order_key(erows[(size_t)i * D + k]));
   }
-  if (tid == 0) s_any = 0;
-  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < D; ++k) atomicMin(&minkey[k], best[k]);
+}
 
-  for (uint32_t b = tid; b < m; b += kBlock) {
-    uint32_t lt = 0, eq = 0;
+// Rank of the bits of `sub` inside `zm` (a parallel bit extract): bit j of
+// sub, j in zm, goes to bit popcount(zm below j).
+__device__ __forceinline__ uint32_t zcompress(uint32_t sub, uint32_t zm) {
+  uint32_t idx = 0;
+  for (uint32_t t = sub; t; t &= t - 1u) {
+    const int bit = __ffs(t) - 1;
+    idx |= 1u << __popc(zm & ((1u << bit) - 1u));
+  }
+  return idx;
+}
+
+// Sets bit `s` of an LDS bitmap unless it is already set.
+__device__ __forceinline__ void lds_mark(uint32_t* w, uint32_t s) {
+  const uint32_t bit = 1u << (s & 31);
+  if (!(w[s >> 5] & bit)) atomicOr(&w[s >> 5], bit);
+}
+
+constexpr int kZMax = 10;  // boundary columns tracked per point (2^10 words)
+
+// Points per workgroup of k_member: as many as keep its LDS near 64 KB.
+template <int D>
+constexpr int member_points() {
+  return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
+}
+
+// Per-point cuboid membership, P extended-skyline points per workgroup (the
+// contract of produce_bitmap_iterative_multi_partition_templated_async,
+// src/utilities/kernels.cuh:415-908, without its approximations).  Every q
+// row is read once per workgroup and tested against the P points.
+//
+// For a point p and a candidate q: lt = {i: q_i < p_i}, eq = {i: q_i ==
+// p_i}, L = lt|eq.  q dominates p in s iff s is inside L and s meets lt.
+// Boundary columns Z = {j: p_j is the column minimum} (at most kZMax of them)
+// admit no q_j < p_j, so for j in Z, j in L iff j in eq.  Write s = sZ + sR
+// (sR outside Z) and eR = eq outside Z (the residual ties):
+//   sR empty          -> never dominated (no strict column inside s)
+//   sR = {j}          -> dominated iff some q has q_j < p_j and ties p on
+//                        all of sZ: bit j of the superset-OR transform U over
+//                        Z's lattice, U[zc(eq & Z)] |= lt  (exact, ties or not)
+//   |sR| >= 2         -> A0 = L of pairs with eR empty (they dominate every s
+//                        inside L with sR non-empty); A1 = L of every pair;
+//                        TE = eR of the others.  After closures, s outside
+//                        T (TE closed down on R, up on Z) follows A1 exactly,
+//                        s in T and in A0 is dominated, s outside A1 is not;
+//                        the rest, X, is decided by a second pass over the
+//                        pairs with residual ties (rare).
+// Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
+template <int D, int P>
+__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows, uint32_t m,
+                                                   int max_d, const uint32_t* __restrict__ minkey,
+                                                   uint32_t* __restrict__ dom) {
+  constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
+  constexpr int ZB = D < kZMax ? D : kZMax;
+  constexpr int NU = 1 << ZB;
+  __shared__ uint32_t A0[P][W];
+  __shared__ uint32_t A1[P][W];
+  __shared__ uint32_t TE[P][W];
+  __shared__ uint32_t U[P][NU];
+  __shared__ uint16_t xl[kXCap];
+  __shared__ int s_any[P];
+  __shared__ uint32_t s_zm[P];
+  __shared__ uint32_t s_rgt[P];
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  const uint32_t base = blockIdx.x * P;
+
+  float p[P][D];
+  uint32_t zm[P], rgt[P];  // tracked boundary columns; columns with p_j > min
+#pragma unroll
+  for (int pp = 0; pp < P; ++pp) {
+    const uint32_t a = min(base + pp, m - 1u);
+    uint32_t z = 0, g = 0;
+    int nz = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const float q = erows[(size_t)b * D + k];
-      lt |= (uint32_t)(q < p[k]) << k;
-      eq |= (uint32_t)(q == p[k]) << k;
+      p[pp][k] = erows[(size_t)a * D + k];
+      const bool at_min = order_key(p[pp][k]) == minkey[k];
+      if (at_min && nz < ZB) {
+        z |= 1u << k;
+        ++nz;
+      }
+      g |= (uint32_t)(!at_min) << k;
     }
-    if (lt) {
+    zm[pp] = z;
+    rgt[pp] = g;
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) {
+      s_zm[pp] = zm[pp];
+      s_rgt[pp] = rgt[pp];
+    }
+  }
+  for (int k = tid; k < W; k += kBlock) {
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) {
+      A0[pp][k] = 0;
+      A1[pp][k] = 0;
+      TE[pp][k] = 0;
+    }
+  }
+  for (int k = tid; k < NU; k += kBlock) {
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) U[pp][k] = 0;
+  }
+  if (tid < P) s_any[tid] = 0;
+  __syncthreads();
+
+  // pass 1: every q against the P points
+  for (uint32_t b = tid; b < m; b += kBlock) {
+    float q[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) q[k] = erows[(size_t)b * D + k];
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) {
+      uint32_t lt = 0, eq = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        lt |= (uint32_t)(q[k] < p[pp][k]) << k;
+        eq |= (uint32_t)(q[k] == p[pp][k]) << k;
+      }
+      if (!lt) continue;
       const uint32_t le = lt | eq;
-      const uint32_t bit = 1u << (le & 31);
-      if (!eq) {
-        if (!(A0[le >> 5] & bit)) atomicOr(&A0[le >> 5], bit);
+      const uint32_t er = eq & ~zm[pp];
+      if (!er) {
+        lds_mark(A0[pp], le);
       } else {
-        if (!(A1[le >> 5] & bit)) atomicOr(&A1[le >> 5], bit);
-        const uint32_t ebit = 1u << (eq & 31);
-        if (!(TE[eq >> 5] & ebit)) atomicOr(&TE[eq >> 5], ebit);
+        lds_mark(A1[pp], le);
+        lds_mark(TE[pp], er);
+      }
+      const uint32_t ez = eq & zm[pp];
+      if (ez) {
+        const uint32_t idx = zcompress(ez, zm[pp]);
+        if ((U[pp][idx] & lt) != lt) atomicOr(&U[pp][idx], lt);
       }
     }
   }
   __syncthreads();
 
+  // closures: A0 and A1 = A0|A1 down; TE down on R, up on Z; U superset-OR
   for (int k = tid; k < W; k += kBlock) {
-    const uint32_t a0 = A0[k];
-    A0[k] = low_closure(a0, D);
-    A1[k] = low_closure(A1[k] | a0, D);
-    TE[k] = low_closure(TE[k], D);
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) {
+      const uint32_t a0 = A0[pp][k];
+      A0[pp][k] = low_closure(a0, D);
+      A1[pp][k] = low_closure(A1[pp][k] | a0, D);
+      uint32_t t = TE[pp][k];
+      const uint32_t z = zm[pp];
+      if (D > 0) t |= (z & 1u) ? (t & 0x55555555u) << 1 : (t & 0xAAAAAAAAu) >> 1;
+      if (D > 1) t |= (z & 2u) ? (t & 0x33333333u) << 2 : (t & 0xCCCCCCCCu) >> 2;
+      if (D > 2) t |= (z & 4u) ? (t & 0x0F0F0F0Fu) << 4 : (t & 0xF0F0F0F0u) >> 4;
+      if (D > 3) t |= (z & 8u) ? (t & 0x00FF00FFu) << 8 : (t & 0xFF00FF00u) >> 8;
+      if (D > 4) t |= (z & 16u) ? (t & 0x0000FFFFu) << 16 : (t & 0xFFFF0000u) >> 16;
+      TE[pp][k] = t;
+    }
   }
 #pragma unroll 1
   for (int j = 5; j < D; ++j) {
@@ -303,32 +410,79 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
     for (int t = tid; t < W / 2; t += kBlock) {
       const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
       const int hi = lo | (1 << sh);
-      A0[lo] |= A0[hi];
-      A1[lo] |= A1[hi];
-      TE[lo] |= TE[hi];
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp) {
+        A0[pp][lo] |= A0[pp][hi];
+        A1[pp][lo] |= A1[pp][hi];
+        if (zm[pp] >> j & 1u)
+          TE[pp][hi] |= TE[pp][lo];
+        else
+          TE[pp][lo] |= TE[pp][hi];
+      }
+    }
+  }
+#pragma unroll 1
+  for (int j = 0; j < ZB; ++j) {
+    __syncthreads();
+    for (int i = tid; i < NU / 2; i += kBlock) {
+      const int lo = ((i >> j) << (j + 1)) | (i & ((1 << j) - 1));
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp)
+        if (j < __popc(zm[pp])) U[pp][lo] |= U[pp][lo | (1 << j)];
     }
   }
   __syncthreads();
 
-  // X = T & D_over & ~D0 within scope; drop X from D_over (re-added below
-  // where exact), keep X in TE.
-  for (int k = tid; k < W; k += kBlock) {
-    const uint32_t x = TE[k] & A1[k] & ~A0[k] & scope_mask(k, D, max_d);
-    TE[k] = x;
-    A1[k] &= ~x;
-    if (x) s_any = 1;
+  // per point: final words; X bits are parked in TE for the second pass
+#pragma unroll 1
+  for (int pp = 0; pp < P; ++pp) {
+    const uint32_t z = s_zm[pp];
+    const uint32_t rg = s_rgt[pp];
+    const uint32_t lowR = ~z & 31u;
+    uint32_t m0 = 0, m1 = 0;  // bits b whose low columns outside Z number 0 / 1
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      const int c = __popc((uint32_t)b & lowR);
+      m0 |= (uint32_t)(c == 0) << b;
+      m1 |= (uint32_t)(c == 1) << b;
+    }
+    for (int k = tid; k < W; k += kBlock) {
+      const int ph = __popc(((uint32_t)k << 5) & ~z);
+      const uint32_t r0 = ph == 0 ? m0 : 0u;
+      const uint32_t r1 = ph == 0 ? m1 : (ph == 1 ? m0 : 0u);
+      const uint32_t r2 = ~(r0 | r1);
+      const uint32_t a0 = A0[pp][k], a1 = A1[pp][k], t = TE[pp][k];
+      const uint32_t sc = scope_mask(k, D, max_d);
+      uint32_t w = ((a1 & ~t) | (a0 & t)) & r2;
+      const uint32_t x = t & a1 & ~a0 & r2 & sc;
+      for (uint32_t bits = r1 & sc; bits; bits &= bits - 1u) {
+        const int b = __ffs(bits) - 1;
+        const uint32_t s = ((uint32_t)k << 5) | (uint32_t)b;
+        const uint32_t sr = s & ~z, sz = s & z;
+        const int j = __ffs(sr) - 1;
+        const uint32_t v = sz ? (U[pp][zcompress(sz, z)] >> j) & 1u : (rg >> j) & 1u;
+        w |= v << b;
+      }
+      A1[pp][k] = (w & sc) | ~sc;
+      TE[pp][k] = x;
+      if (x) s_any[pp] = 1;
+    }
   }
   __syncthreads();
 
-  if (s_any) {  // uniform
-    for (int k = tid; k < W; k += kBlock) A0[k] = 0;  // exact bits of X
+  // second pass for X (residual-tie subspaces): uniform per point
+#pragma unroll 1
+  for (int pp = 0; pp < P; ++pp) {
+    if (!s_any[pp]) continue;
+    const uint32_t z = s_zm[pp];
+    const float* prow = erows + (size_t)min(base + pp, m - 1u) * D;
 #pragma unroll 1
     while (true) {
       __syncthreads();
       if (tid == 0) s_cnt = 0;
       __syncthreads();
       for (int k = tid; k < W; k += kBlock) {
-        uint32_t x = TE[k];
+        uint32_t x = TE[pp][k];
         while (x) {
           const int bpos = __ffs(x) - 1;
           const int slot = atomicAdd(&s_cnt, 1);
@@ -336,36 +490,35 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
           xl[slot] = (uint16_t)((k << 5) | bpos);
           x &= x - 1u;
         }
-        TE[k] = x;
+        TE[pp][k] = x;
       }
       __syncthreads();
       const int cnt = min(s_cnt, kXCap);
-      if (cnt == 0) break;  // uniform
+      if (cnt == 0) break;
       for (uint32_t b = tid; b < m; b += kBlock) {
         uint32_t lt = 0, eq = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           const float q = erows[(size_t)b * D + k];
-          lt |= (uint32_t)(q < p[k]) << k;
-          eq |= (uint32_t)(q == p[k]) << k;
+          const float pv = prow[k];
+          lt |= (uint32_t)(q < pv) << k;
+          eq |= (uint32_t)(q == pv) << k;
         }
-        if (lt == 0 || eq == 0) continue;
+        if (lt == 0 || (eq & ~z) == 0) continue;
         const uint32_t le = lt | eq;
         for (int e = 0; e < cnt; ++e) {
           const uint32_t s = xl[e];
-          if (!(s & ~le) && (s & lt)) atomicOr(&A0[s >> 5], 1u << (s & 31));
+          if (!(s & ~le) && (s & lt)) lds_mark(A1[pp], s);
         }
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
-  const int any = s_any;
-  for (int k = tid; k < W; k += kBlock) {
-    uint32_t w = A1[k];
-    if (any) w |= A0[k];
-    w |= ~scope_mask(k, D, max_d);
-    dom[(size_t)a * W + k] = w;
+#pragma unroll
+  for (int pp = 0; pp < P; ++pp) {
+    if (base + pp >= m) break;
+    for (int k = tid; k < W; k += kBlock) dom[(size_t)(base + pp) * W + k] = A1[pp][k];
   }
 }
 

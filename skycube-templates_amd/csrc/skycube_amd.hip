@@ -38,6 +38,10 @@ struct sky_ctx {
   // device buffers
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand;
+  // stage timing (HIP events on `stream`)
+  bool timing = false;
+  hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
+  float stage_ms[SKY_STAGE_COUNT] = {};
   // last MDMC result
   bool has_mdmc = false;
   uint32_t res_n_ext = 0, res_d = 0, res_W = 0;
@@ -103,6 +107,23 @@ int ensure(sky_ctx* c, DevBuf& b, size_t bytes) {
 template <typename T>
 T* P(DevBuf& b) {
   return static_cast<T*>(b.p);
+}
+
+// Records the event that opens stage `k` (stage k-1 ends there).
+int mark(sky_ctx* c, int k) {
+  if (!c->timing) return SKY_OK;
+  if (!c->ev[k]) HIPCHK(hipEventCreate(&c->ev[k]));
+  HIPCHK(hipEventRecord(c->ev[k], c->stream));
+  return SKY_OK;
+}
+
+int read_marks(sky_ctx* c) {
+  if (!c->timing) return SKY_OK;
+  for (int k = 0; k < SKY_STAGE_COUNT; ++k) {
+    c->stage_ms[k] = 0.0f;
+    if (c->ev[k] && c->ev[k + 1]) HIPCHK(hipEventElapsedTime(&c->stage_ms[k], c->ev[k], c->ev[k + 1]));
+  }
+  return SKY_OK;
 }
 
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -180,8 +201,9 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
   TRY(ensure(c, c->s1keys, sizeof(uint32_t) * nc));
   TRY(ensure(c, c->s1pid, sizeof(uint32_t) * nc));
   TRY(ensure(c, c->status, nc));
-  TRY(ensure(c, c->counter, 4 * sizeof(uint32_t)));
+  TRY(ensure(c, c->counter, 64 * sizeof(uint32_t)));
 
+  TRY(mark(c, SKY_STAGE_SCORE_SORT));
   HIPCHK(hipMemsetAsync(c->counter.p, 0, 4 * sizeof(uint32_t), c->stream));
   uint32_t* err_dev = P<uint32_t>(c->counter) + 1;
   hipLaunchKernelGGL(sky::k_score, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, rows_dev,
@@ -208,6 +230,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
     hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream,
                        rows_dev, d, P<uint32_t>(c->pid_b), nc, dims, P<float>(c->srows));
     LAUNCHCHK();
+    TRY(mark(c, SKY_STAGE_PRUNE));
     hipLaunchKernelGGL(sky::k_prune<D>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream,
                        P<float>(c->srows), P<uint32_t>(c->key_b), nc, npr, P<uint8_t>(c->alive));
     LAUNCHCHK();
@@ -222,6 +245,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
                        P<uint32_t>(c->s1idx), m, P<float>(c->s1rows), P<uint32_t>(c->s1keys),
                        P<uint32_t>(c->s1pid));
     LAUNCHCHK();
+    TRY(mark(c, SKY_STAGE_SELF));
     hipLaunchKernelGGL(sky::k_self<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
                        P<float>(c->s1rows), P<uint32_t>(c->s1keys), m, need_dom,
                        P<uint8_t>(c->status));
@@ -253,6 +277,7 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
   }
   uint32_t m = 0;
   TRY(run_split(c, rows_dev, n, d, (1u << d) - 1u, nullptr, n, 0, &m));
+  TRY(mark(c, SKY_STAGE_COMPACT));
   TRY(ensure(c, c->flag, n));
   TRY(ensure(c, c->ext_pids, sizeof(uint32_t) * n));
   HIPCHK(hipMemsetAsync(c->flag.p, 0, n, c->stream));
@@ -266,15 +291,29 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
   TRY(ensure(c, c->dom, sizeof(uint32_t) * (size_t)ne * W));
   uint32_t ds = 0;
   const DimList dims = dims_of((1u << d) - 1u, &ds);
-  SKY_DISPATCH_D(d, {
-    hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
-                       rows_dev, d, P<uint32_t>(c->ext_pids), ne, dims, P<float>(c->erows));
-    LAUNCHCHK();
-    hipLaunchKernelGGL(sky::k_member<D>, dim3(ne), dim3(kBlock), 0, c->stream,
-                       P<float>(c->erows), ne, (int)max_d, P<uint32_t>(c->dom));
-    LAUNCHCHK();
-  });
+  uint32_t* minkey = P<uint32_t>(c->counter) + 16;
+  HIPCHK(hipMemsetAsync(minkey, 0xFF, 16 * sizeof(uint32_t), c->stream));
+  if (ne) {
+    SKY_DISPATCH_D(d, {
+      hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                         rows_dev, d, P<uint32_t>(c->ext_pids), ne, dims, P<float>(c->erows));
+      LAUNCHCHK();
+      hipLaunchKernelGGL(sky::k_colmin<D>, dim3(std::min<unsigned>(grid_for(ne), 1024u)),
+                         dim3(kBlock), 0, c->stream, P<float>(c->erows), ne, minkey);
+      LAUNCHCHK();
+      TRY(mark(c, SKY_STAGE_MEMBER));
+      constexpr int PP = sky::member_points<D>();
+      hipLaunchKernelGGL((sky::k_member<D, PP>), dim3((ne + PP - 1) / PP), dim3(kBlock), 0,
+                         c->stream, P<float>(c->erows), ne, (int)max_d, minkey,
+                         P<uint32_t>(c->dom));
+      LAUNCHCHK();
+    });
+  } else {
+    TRY(mark(c, SKY_STAGE_MEMBER));
+  }
+  TRY(mark(c, SKY_STAGE_COUNT));
   HIPCHK(hipStreamSynchronize(c->stream));
+  TRY(read_marks(c));
   c->has_mdmc = true;
   c->res_n_ext = ne;
   c->res_d = d;
@@ -355,6 +394,8 @@ void sky_ctx_destroy(sky_ctx* c) {
                     &c->dom,    &c->cub_tmp, &c->counter, &c->cand};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -362,6 +403,18 @@ void sky_ctx_destroy(sky_ctx* c) {
 const char* sky_last_error(const sky_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 size_t sky_ctx_bytes_in_use(const sky_ctx* c) { return c ? c->in_use : 0; }
+
+int sky_ctx_set_timing(sky_ctx* c, int enable) {
+  if (!c) return SKY_ERR_ARG;
+  c->timing = enable != 0;
+  return SKY_OK;
+}
+
+int sky_ctx_stage_times(const sky_ctx* c, float* ms, int count) {
+  if (!c || !ms || count < 0) return SKY_ERR_ARG;
+  for (int k = 0; k < count && k < SKY_STAGE_COUNT; ++k) ms[k] = c->stage_ms[k];
+  return SKY_OK;
+}
 
 int sky_mdmc_device(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
                     uint32_t* n_ext) {

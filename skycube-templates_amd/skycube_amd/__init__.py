@@ -34,8 +34,10 @@ _ERRORS = {
 EXPORTS = (
     "sky_ctx_create", "sky_ctx_destroy", "sky_last_error", "sky_words_per_point",
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
-    "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc",
+    "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
+    "sky_ctx_stage_times",
 )
+STAGES = ("score_sort", "prune", "self", "compact", "member")
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -85,6 +87,8 @@ def load():
     L.sky_sdsc.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                            ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p,
                            ctypes.c_uint64]
+    L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     _lib = L
     return L
 
@@ -156,6 +160,16 @@ class Context:
     def _check(self, rc: int):
         if rc != SKY_OK:
             raise SkyError(rc, self._L.sky_last_error(self._h).decode(errors="replace"))
+
+    def set_timing(self, enable: bool = True) -> None:
+        self._check(self._L.sky_ctx_set_timing(self._h, int(bool(enable))))
+
+    def stage_times(self) -> dict:
+        """Device ms per stage of the last MDMC call (HIP events on the
+        context's stream; needs set_timing(True))."""
+        ms = np.zeros(len(STAGES), dtype=np.float32)
+        self._check(self._L.sky_ctx_stage_times(self._h, _ptr(ms, _f32p), ms.size))
+        return {k: float(v) for k, v in zip(STAGES, ms)}
 
     def bytes_in_use(self) -> int:
         return int(self._L.sky_ctx_bytes_in_use(self._h))

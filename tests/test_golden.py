@@ -1,0 +1,96 @@
+"""Golden fixtures (tests/golden/): the reference's NBA dataset and seeded
+workload samples, plus sha256 digests of the three full-size workloads.
+
+CPU tests pin the oracle to the fixtures; GPU tests hold the HIP path to them.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from skycube_amd import datagen
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+META = json.load(open(os.path.join(GOLD, "golden.json")))
+
+
+def _nba_rows():
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import read_csv_strtof
+    return read_csv_strtof(os.path.join(GOLD, "nba-U-8-17264.csv.gz"))
+
+
+def _digest(ext, dom):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(ext, dtype="<u4").tobytes())
+    h.update(np.ascontiguousarray(dom, dtype="<u4").tobytes())
+    return h.hexdigest()
+
+
+def _samples():
+    return [(name, m["workload"], m["n"], m["max_d"]) for name, m in sorted(META["samples"].items())]
+
+
+def test_nba_input_shape():
+    x = _nba_rows()
+    assert x.shape == (17264, 8)
+    assert x.dtype == np.float32
+    # per-column ties are present in this dataset
+    assert min(np.unique(x[:, j]).size for j in range(8)) < 17264
+
+
+def test_oracle_reproduces_nba():
+    from oracle import skyorc
+    ext, dom = skyorc.mdmc(_nba_rows(), 8)
+    ref = np.load(os.path.join(GOLD, "nba_mdmc.npz"))
+    assert np.array_equal(ext, ref["ext"]) and np.array_equal(dom, ref["dom"])
+    assert _digest(ext, dom) == META["nba"]["sha256"]
+
+
+@pytest.mark.parametrize("name,wl,n,max_d", _samples())
+def test_oracle_reproduces_samples(name, wl, n, max_d):
+    from oracle import skyorc
+    ext, dom = skyorc.mdmc(datagen.workload(wl, n), max_d)
+    assert _digest(ext, dom) == META["samples"][name]["sha256"]
+
+
+def test_fixture_score_ties_absent():
+    """No fixture input holds a (q, p, s) with q dominating p in s and equal
+    FP32 Manhattan norms, so STSC's score early exit cannot make STSC differ
+    from these sets."""
+    assert META["nba"]["score_tie_triples"] == 0
+    for m in META["samples"].values():
+        assert m["score_tie_triples"] == 0
+    for m in META.get("full", {}).values():
+        assert m["score_tie_triples"] == 0
+
+
+@pytest.mark.gpu
+def test_gpu_nba(ctx):
+    ext, dom = ctx.mdmc(_nba_rows(), 8)
+    assert _digest(ext, dom) == META["nba"]["sha256"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,wl,n,max_d", _samples())
+def test_gpu_samples(ctx, name, wl, n, max_d):
+    ext, dom = ctx.mdmc(datagen.workload(wl, n), max_d)
+    assert _digest(ext, dom) == META["samples"][name]["sha256"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wl", sorted(datagen.WORKLOADS))
+def test_gpu_full_workload_digest(ctx, wl):
+    """Bit-exact at full size: n_ext and the sha256 of (ext pids, bitmaps)."""
+    full = META.get("full", {})
+    if wl not in full:
+        pytest.skip("full-size digest not generated")
+    m = full[wl]
+    print(f"full workload {wl} n={m['n']} d={m['d']} max_d={m['max_d']}", flush=True)
+    ext, dom = ctx.mdmc(datagen.workload(wl), m["max_d"])
+    assert ext.size == m["n_ext"]
+    assert _digest(ext, dom) == m["sha256"]
