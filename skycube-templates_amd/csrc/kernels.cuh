@@ -203,6 +203,40 @@ __global__ void k_mark_ext(const uint8_t* __restrict__ status, const uint32_t* _
   if (j < m && status[j] != 2) flag[pid[j]] = 1;
 }
 
+// flag[j] = 1 for every survivor j that nothing strictly dominates.
+__global__ void k_flag_ext(const uint8_t* __restrict__ status, uint32_t m,
+                           uint8_t* __restrict__ flag) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) flag[j] = status[j] != 2;
+}
+
+// The extended skyline in score order: rows and ids at survivor positions.
+template <int D>
+__global__ void k_gather_ext(const float* __restrict__ s1rows, const uint32_t* __restrict__ s1pid,
+                             const uint32_t* __restrict__ idx, uint32_t ne,
+                             float* __restrict__ erows, uint32_t* __restrict__ epid) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ne) return;
+  const uint32_t i = idx[j];
+#pragma unroll
+  for (int k = 0; k < D; ++k) erows[(size_t)j * D + k] = s1rows[(size_t)i * D + k];
+  epid[j] = s1pid[i];
+}
+
+// rank[ids[i]] = i (ids ascending point ids of the extended skyline).
+__global__ void k_scatter_rank(const uint32_t* __restrict__ ids, uint32_t ne,
+                               uint32_t* __restrict__ rank) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) rank[ids[i]] = i;
+}
+
+// pos[j] = output row (ascending-id position) of score-order point j.
+__global__ void k_pos(const uint32_t* __restrict__ epid, uint32_t ne,
+                      const uint32_t* __restrict__ rank, uint32_t* __restrict__ pos) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < ne) pos[j] = rank[epid[j]];
+}
+
 // ---------------------------------------------------------------------------
 // Lattice helpers.  A bitmap over the 2^D subspaces is W = max(1, 2^D/32)
 // words; subspace s is bit s&31 of word s>>5.
@@ -264,7 +298,9 @@ __device__ __forceinline__ void lds_mark(uint32_t* w, uint32_t s) {
   if (!(w[s >> 5] & bit)) atomicOr(&w[s >> 5], bit);
 }
 
-constexpr int kZMax = 10;  // boundary columns tracked per point (2^10 words)
+constexpr int kZMax = 10;     // boundary columns tracked per point (2^10 words)
+constexpr int kCodePivots = 7;  // 8 quantile buckets per column, 3-bit codes
+constexpr uint32_t kCover = 2048;  // lowest-score candidates scanned first
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB.
 template <int D>
@@ -272,10 +308,126 @@ constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
 }
 
+// Quantile pivots of each column of the extended skyline, as order keys:
+// block = one column; a strided sample of up to 2048 values is ranked by
+// counting and the values at ranks b*S/8 (b = 1..7) become the pivots.  Any
+// pivots give correct codes; good ones make the codes selective.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ erows, uint32_t m,
+                                                   uint32_t* __restrict__ piv) {
+  constexpr int S = 2048;
+  __shared__ uint32_t val[S];
+  __shared__ uint32_t srt[S];
+  const int col = blockIdx.x;
+  const uint32_t cnt = min((uint32_t)S, m);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
+    val[i] = order_key(erows[(size_t)((uint64_t)i * m / cnt) * D + col]);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    const uint32_t v = val[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < cnt; ++j) r += (val[j] < v) || (val[j] == v && j < i);
+    srt[r] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kCodePivots)
+    piv[col * kCodePivots + threadIdx.x] = srt[(threadIdx.x + 1) * cnt / (kCodePivots + 1)];
+}
+
+// 3-bit quantile code of every column, 8 columns per word (nibbles); the
+// code is the number of pivots below the value, so v <= w implies
+// code(v) <= code(w).
+template <int D>
+__global__ void k_codes(const float* __restrict__ erows, uint32_t m,
+                        const uint32_t* __restrict__ piv, uint2* __restrict__ codes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t w[2] = {0u, 0u};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const uint32_t key = order_key(erows[(size_t)i * D + k]);
+    uint32_t c = 0;
+#pragma unroll
+    for (int b = 0; b < kCodePivots; ++b) c += piv[k * kCodePivots + b] < key;
+    w[k >> 3] |= c << ((k & 7) * 4);
+  }
+  codes[i] = make_uint2(w[0], w[1]);
+}
+
+// Bit k (k < 8) is set iff nibble k of q is <= nibble k of p (3-bit values):
+// the guard bit of each nibble of (p | 8..8) - q survives iff p >= q; the
+// eight guard bits are then folded into one byte.
+__device__ __forceinline__ uint32_t code_le(uint32_t q, uint32_t p) {
+  const uint32_t x = (p | 0x88888888u) - q;
+  uint32_t g = (x >> 3) & 0x11111111u;
+  g = (g | (g >> 3)) & 0x03030303u;
+  g = (g | (g >> 6)) & 0x000F000Fu;
+  return (g | (g >> 12)) & 0xFFu;
+}
+
+// Marks one (q, p) pair with masks lt, eq into the structures of point pp.
+template <int W, int NU>
+__device__ __forceinline__ void mark_pair(uint32_t (*A0)[W], uint32_t (*A1)[W], uint32_t (*TE)[W],
+                                          uint32_t (*U)[NU], int pp, uint32_t lt, uint32_t eq,
+                                          uint32_t z) {
+  if (!lt) return;
+  const uint32_t le = lt | eq;
+  const uint32_t er = eq & ~z;
+  if (!er) {
+    lds_mark(A0[pp], le);
+  } else {
+    lds_mark(A1[pp], le);
+    lds_mark(TE[pp], er);
+  }
+  const uint32_t ez = eq & z;
+  if (ez) {
+    const uint32_t idx = zcompress(ez, z);
+    if ((U[pp][idx] & lt) != lt) atomicOr(&U[pp][idx], lt);
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void pair_masks(const float* __restrict__ q, const float* p,
+                                           uint32_t& lt, uint32_t& eq) {
+  float qv[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) qv[k] = q[k];
+  lt = 0;
+  eq = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    lt |= (uint32_t)(qv[k] < p[k]) << k;
+    eq |= (uint32_t)(qv[k] == p[k]) << k;
+  }
+}
+
+// Down-closure of the P bitmaps A (W words each) in place; caller syncs
+// before.  Ends with a barrier.
+template <int D, int P, int W>
+__device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
+  for (int k = threadIdx.x; k < W; k += kBlock) {
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) A[pp][k] = low_closure(A[pp][k], D);
+  }
+#pragma unroll 1
+  for (int j = 5; j < D; ++j) {
+    __syncthreads();
+    const int sh = j - 5;
+    for (int t = threadIdx.x; t < W / 2; t += kBlock) {
+      const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp) A[pp][lo] |= A[pp][lo | (1 << sh)];
+    }
+  }
+  __syncthreads();
+}
+
 // Per-point cuboid membership, P extended-skyline points per workgroup (the
 // contract of produce_bitmap_iterative_multi_partition_templated_async,
-// src/utilities/kernels.cuh:415-908, without its approximations).  Every q
-// row is read once per workgroup and tested against the P points.
+// src/utilities/kernels.cuh:415-908, without its approximations).
+//
+// Points and candidates are the extended skyline in score order; point a's
+// bitmap goes to row pos[a] of dom (ascending point id).
 //
 // For a point p and a candidate q: lt = {i: q_i < p_i}, eq = {i: q_i ==
 // p_i}, L = lt|eq.  q dominates p in s iff s is inside L and s meets lt.
@@ -293,52 +445,58 @@ constexpr int member_points() {
 //                        s in T and in A0 is dominated, s outside A1 is not;
 //                        the rest, X, is decided by a second pass over the
 //                        pairs with residual ties (rare).
+//
+// Pruning: the kCover lowest-score candidates are marked first and A0 is
+// closed.  Any set bit of A0 is then inside the L of a marked pair with no
+// residual tie, which dominates p in every subspace (sR non-empty) that a
+// pair with a smaller L could, and carries its U contribution too.  So a
+// later q whose code bound on L (columns with code_q <= code_p, a superset
+// of L) is set in A0 adds nothing and is skipped.  Survivors of the code
+// test are compacted into an LDS queue and get the exact FP32 test together.
 // Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
 template <int D, int P>
-__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows, uint32_t m,
+__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
+                                                   const uint2* __restrict__ codes,
+                                                   const uint32_t* __restrict__ pos, uint32_t m,
                                                    int max_d, const uint32_t* __restrict__ minkey,
                                                    uint32_t* __restrict__ dom) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = D < kZMax ? D : kZMax;
   constexpr int NU = 1 << ZB;
+  constexpr int QCAP = kBlock * (P + 1);
+  constexpr uint32_t FULL = (1u << D) - 1u;
   __shared__ uint32_t A0[P][W];
   __shared__ uint32_t A1[P][W];
   __shared__ uint32_t TE[P][W];
   __shared__ uint32_t U[P][NU];
-  __shared__ uint16_t xl[kXCap];
+  __shared__ float prow[P][D];
+  __shared__ uint32_t queue[QCAP];
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
   __shared__ int s_cnt;
+  uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
+  constexpr int XCAP = QCAP * 2 < kXCap ? QCAP * 2 : kXCap;
   const int tid = threadIdx.x;
   const uint32_t base = blockIdx.x * P;
 
-  float p[P][D];
-  uint32_t zm[P], rgt[P];  // tracked boundary columns; columns with p_j > min
-#pragma unroll
-  for (int pp = 0; pp < P; ++pp) {
-    const uint32_t a = min(base + pp, m - 1u);
+  if (tid < P) {
+    const uint32_t a = min(base + tid, m - 1u);
     uint32_t z = 0, g = 0;
     int nz = 0;
-#pragma unroll
     for (int k = 0; k < D; ++k) {
-      p[pp][k] = erows[(size_t)a * D + k];
-      const bool at_min = order_key(p[pp][k]) == minkey[k];
+      const float v = erows[(size_t)a * D + k];
+      prow[tid][k] = v;
+      const bool at_min = order_key(v) == minkey[k];
       if (at_min && nz < ZB) {
         z |= 1u << k;
         ++nz;
       }
       g |= (uint32_t)(!at_min) << k;
     }
-    zm[pp] = z;
-    rgt[pp] = g;
-  }
-  if (tid == 0) {
-#pragma unroll
-    for (int pp = 0; pp < P; ++pp) {
-      s_zm[pp] = zm[pp];
-      s_rgt[pp] = rgt[pp];
-    }
+    s_zm[tid] = z;
+    s_rgt[tid] = g;
+    s_any[tid] = 0;
   }
   for (int k = tid; k < W; k += kBlock) {
 #pragma unroll
@@ -352,49 +510,63 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) U[pp][k] = 0;
   }
-  if (tid < P) s_any[tid] = 0;
+  if (tid == 0) s_cnt = 0;
   __syncthreads();
 
-  // pass 1: every q against the P points
-  for (uint32_t b = tid; b < m; b += kBlock) {
-    float q[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) q[k] = erows[(size_t)b * D + k];
-#pragma unroll
-    for (int pp = 0; pp < P; ++pp) {
-      uint32_t lt = 0, eq = 0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        lt |= (uint32_t)(q[k] < p[pp][k]) << k;
-        eq |= (uint32_t)(q[k] == p[pp][k]) << k;
-      }
-      if (!lt) continue;
-      const uint32_t le = lt | eq;
-      const uint32_t er = eq & ~zm[pp];
-      if (!er) {
-        lds_mark(A0[pp], le);
-      } else {
-        lds_mark(A1[pp], le);
-        lds_mark(TE[pp], er);
-      }
-      const uint32_t ez = eq & zm[pp];
-      if (ez) {
-        const uint32_t idx = zcompress(ez, zm[pp]);
-        if ((U[pp][idx] & lt) != lt) atomicOr(&U[pp][idx], lt);
-      }
-    }
+  // pass 1a: the lowest-score candidates, every pair tested
+  const uint32_t kc = min(m, kCover);
+  for (uint32_t e = tid; e < kc * P; e += kBlock) {
+    const uint32_t b = e / P;
+    const int pp = (int)(e - b * P);
+    uint32_t lt, eq;
+    pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
+    mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
   }
   __syncthreads();
+  close_down<D, P, W>(A0);
+
+  // pass 1b: code test, queue, exact test of the survivors
+  uint2 pc[P];
+#pragma unroll
+  for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
+  for (uint32_t c0 = kc; c0 < m; c0 += kBlock) {
+    const uint32_t b = c0 + tid;
+    if (b < m) {
+      const uint2 qc = codes[b];
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp) {
+        uint32_t ub = code_le(qc.x, pc[pp].x);
+        if (D > 8) ub |= code_le(qc.y, pc[pp].y) << 8;
+        ub &= FULL;
+        if (!((A0[pp][ub >> 5] >> (ub & 31)) & 1u)) {
+          const int slot = atomicAdd(&s_cnt, 1);
+          queue[slot] = (b << 3) | (uint32_t)pp;
+        }
+      }
+    }
+    __syncthreads();
+    const int n = s_cnt;
+    if (n >= kBlock || c0 + kBlock >= m) {  // uniform
+      for (int e = tid; e < n; e += kBlock) {
+        const uint32_t v = queue[e];
+        const int pp = (int)(v & 7u);
+        uint32_t lt, eq;
+        pair_masks<D>(erows + (size_t)(v >> 3) * D, prow[pp], lt, eq);
+        mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
+      }
+      __syncthreads();
+      if (tid == 0) s_cnt = 0;
+    }
+    __syncthreads();
+  }
 
   // closures: A0 and A1 = A0|A1 down; TE down on R, up on Z; U superset-OR
   for (int k = tid; k < W; k += kBlock) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
-      const uint32_t a0 = A0[pp][k];
-      A0[pp][k] = low_closure(a0, D);
-      A1[pp][k] = low_closure(A1[pp][k] | a0, D);
+      A1[pp][k] |= A0[pp][k];
       uint32_t t = TE[pp][k];
-      const uint32_t z = zm[pp];
+      const uint32_t z = s_zm[pp];
       if (D > 0) t |= (z & 1u) ? (t & 0x55555555u) << 1 : (t & 0xAAAAAAAAu) >> 1;
       if (D > 1) t |= (z & 2u) ? (t & 0x33333333u) << 2 : (t & 0xCCCCCCCCu) >> 2;
       if (D > 2) t |= (z & 4u) ? (t & 0x0F0F0F0Fu) << 4 : (t & 0xF0F0F0F0u) >> 4;
@@ -403,35 +575,35 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
       TE[pp][k] = t;
     }
   }
+  __syncthreads();
+  close_down<D, P, W>(A0);
+  close_down<D, P, W>(A1);
 #pragma unroll 1
   for (int j = 5; j < D; ++j) {
-    __syncthreads();
     const int sh = j - 5;
     for (int t = tid; t < W / 2; t += kBlock) {
       const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
       const int hi = lo | (1 << sh);
 #pragma unroll
       for (int pp = 0; pp < P; ++pp) {
-        A0[pp][lo] |= A0[pp][hi];
-        A1[pp][lo] |= A1[pp][hi];
-        if (zm[pp] >> j & 1u)
+        if (s_zm[pp] >> j & 1u)
           TE[pp][hi] |= TE[pp][lo];
         else
           TE[pp][lo] |= TE[pp][hi];
       }
     }
+    __syncthreads();
   }
 #pragma unroll 1
   for (int j = 0; j < ZB; ++j) {
-    __syncthreads();
     for (int i = tid; i < NU / 2; i += kBlock) {
       const int lo = ((i >> j) << (j + 1)) | (i & ((1 << j) - 1));
 #pragma unroll
       for (int pp = 0; pp < P; ++pp)
-        if (j < __popc(zm[pp])) U[pp][lo] |= U[pp][lo | (1 << j)];
+        if (j < __popc(s_zm[pp])) U[pp][lo] |= U[pp][lo | (1 << j)];
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   // per point: final words; X bits are parked in TE for the second pass
 #pragma unroll 1
@@ -475,7 +647,6 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
   for (int pp = 0; pp < P; ++pp) {
     if (!s_any[pp]) continue;
     const uint32_t z = s_zm[pp];
-    const float* prow = erows + (size_t)min(base + pp, m - 1u) * D;
 #pragma unroll 1
     while (true) {
       __syncthreads();
@@ -486,24 +657,18 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
         while (x) {
           const int bpos = __ffs(x) - 1;
           const int slot = atomicAdd(&s_cnt, 1);
-          if (slot >= kXCap) break;
+          if (slot >= XCAP) break;
           xl[slot] = (uint16_t)((k << 5) | bpos);
           x &= x - 1u;
         }
         TE[pp][k] = x;
       }
       __syncthreads();
-      const int cnt = min(s_cnt, kXCap);
+      const int cnt = min(s_cnt, XCAP);
       if (cnt == 0) break;
       for (uint32_t b = tid; b < m; b += kBlock) {
-        uint32_t lt = 0, eq = 0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          const float q = erows[(size_t)b * D + k];
-          const float pv = prow[k];
-          lt |= (uint32_t)(q < pv) << k;
-          eq |= (uint32_t)(q == pv) << k;
-        }
+        uint32_t lt, eq;
+        pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
         if (lt == 0 || (eq & ~z) == 0) continue;
         const uint32_t le = lt | eq;
         for (int e = 0; e < cnt; ++e) {
@@ -518,7 +683,8 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) {
     if (base + pp >= m) break;
-    for (int k = tid; k < W; k += kBlock) dom[(size_t)(base + pp) * W + k] = A1[pp][k];
+    const uint32_t row = pos[base + pp];
+    for (int k = tid; k < W; k += kBlock) dom[(size_t)row * W + k] = A1[pp][k];
   }
 }
 

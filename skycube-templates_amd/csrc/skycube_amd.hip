@@ -37,7 +37,7 @@ struct sky_ctx {
   std::string err;
   // device buffers
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
-      status, flag, ext_pids, erows, dom, cub_tmp, counter, cand;
+      status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv;
   // stage timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
@@ -289,23 +289,45 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
   TRY(copy_count(c, &ne));
   TRY(ensure(c, c->erows, sizeof(float) * (size_t)ne * d));
   TRY(ensure(c, c->dom, sizeof(uint32_t) * (size_t)ne * W));
-  uint32_t ds = 0;
-  const DimList dims = dims_of((1u << d) - 1u, &ds);
+  TRY(ensure(c, c->eidx, sizeof(uint32_t) * (size_t)ne));
+  TRY(ensure(c, c->epid, sizeof(uint32_t) * (size_t)ne));
+  TRY(ensure(c, c->pos, sizeof(uint32_t) * (size_t)ne));
+  TRY(ensure(c, c->codes, sizeof(uint2) * (size_t)ne));
+  TRY(ensure(c, c->piv, sizeof(uint32_t) * 16 * sky::kCodePivots));
   uint32_t* minkey = P<uint32_t>(c->counter) + 16;
   HIPCHK(hipMemsetAsync(minkey, 0xFF, 16 * sizeof(uint32_t), c->stream));
   if (ne) {
+    // the extended skyline in score order (candidates and points of the
+    // membership kernel) and each point's output row in ascending-id order
+    hipLaunchKernelGGL(sky::k_flag_ext, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                       P<uint8_t>(c->status), m, P<uint8_t>(c->alive));
+    LAUNCHCHK();
+    TRY(select_flagged(c, P<uint8_t>(c->alive), m, P<uint32_t>(c->eidx)));
+    hipLaunchKernelGGL(sky::k_scatter_rank, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(c->ext_pids), ne, P<uint32_t>(c->key_a));
+    LAUNCHCHK();
     SKY_DISPATCH_D(d, {
-      hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
-                         rows_dev, d, P<uint32_t>(c->ext_pids), ne, dims, P<float>(c->erows));
+      hipLaunchKernelGGL(sky::k_gather_ext<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                         P<float>(c->s1rows), P<uint32_t>(c->s1pid), P<uint32_t>(c->eidx), ne,
+                         P<float>(c->erows), P<uint32_t>(c->epid));
+      LAUNCHCHK();
+      hipLaunchKernelGGL(sky::k_pos, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                         P<uint32_t>(c->epid), ne, P<uint32_t>(c->key_a), P<uint32_t>(c->pos));
       LAUNCHCHK();
       hipLaunchKernelGGL(sky::k_colmin<D>, dim3(std::min<unsigned>(grid_for(ne), 1024u)),
                          dim3(kBlock), 0, c->stream, P<float>(c->erows), ne, minkey);
       LAUNCHCHK();
+      hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D), dim3(kBlock), 0, c->stream,
+                         P<float>(c->erows), ne, P<uint32_t>(c->piv));
+      LAUNCHCHK();
+      hipLaunchKernelGGL(sky::k_codes<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                         P<float>(c->erows), ne, P<uint32_t>(c->piv), P<uint2>(c->codes));
+      LAUNCHCHK();
       TRY(mark(c, SKY_STAGE_MEMBER));
       constexpr int PP = sky::member_points<D>();
       hipLaunchKernelGGL((sky::k_member<D, PP>), dim3((ne + PP - 1) / PP), dim3(kBlock), 0,
-                         c->stream, P<float>(c->erows), ne, (int)max_d, minkey,
-                         P<uint32_t>(c->dom));
+                         c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
+                         ne, (int)max_d, minkey, P<uint32_t>(c->dom));
       LAUNCHCHK();
     });
   } else {
@@ -391,7 +413,8 @@ void sky_ctx_destroy(sky_ctx* c) {
   DevBuf* bufs[] = {&c->rows,   &c->key_a,  &c->key_b,    &c->pid_a,  &c->pid_b,
                     &c->srows,  &c->alive,  &c->s1idx,    &c->s1rows, &c->s1keys,
                     &c->s1pid,  &c->status, &c->flag,     &c->ext_pids, &c->erows,
-                    &c->dom,    &c->cub_tmp, &c->counter, &c->cand};
+                    &c->dom,    &c->cub_tmp, &c->counter, &c->cand,  &c->eidx,
+                    &c->epid,   &c->pos,    &c->codes,    &c->piv};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : c->ev)

@@ -46,6 +46,16 @@ def test_mdmc_every_d_continuous(ctx, d):
     check_mdmc(ctx, rng.random((700, d), dtype=np.float32), tag="uniform")
 
 
+@pytest.mark.parametrize("d", [3, 4, 6, 8, 10, 12, 13, 14, 15, 16])
+def test_mdmc_large_ext_with_ties(ctx, d):
+    """More than 2048 extended-skyline points, so the code-pruned scan runs,
+    with ties (anticorrelated values rounded to a coarse grid)."""
+    x = datagen.anticorrelated(7000, d, 6000 + d)
+    x = np.floor(x * 40).astype(np.float32)
+    ext, _ = check_mdmc(ctx, x, tag="anti-ties")
+    assert ext.size > 2048 or d <= 4
+
+
 @pytest.mark.parametrize("d,max_d", [(6, 1), (9, 3), (12, 5), (16, 4)])
 def test_mdmc_max_d(ctx, d, max_d):
     rng = np.random.default_rng(3000 + d)
