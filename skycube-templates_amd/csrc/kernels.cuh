@@ -131,69 +131,86 @@ __global__ __launch_bounds__(kBlock) void k_prune(const float* __restrict__ srow
   if (valid) alive[i] = dead ? 0 : 1;
 }
 
-// Survivors against each other, in score order: status[i] = 2 if strictly
-// dominated, else 1 if dominated (only when need_dom), else 0.  Point i is
-// tested against the survivors with key <= its own; q tiles sit in LDS and
-// are read as broadcasts by the 256 points of the workgroup, which stops as
-// soon as all its points are settled.  One SkyAlign pass of
-// total_pruning_moved_data_templated_pitch_two_level_full_warps
-// (src/utilities/kernels.cuh:36-411) over the whole candidate set.
+// Median mask of each row: bit k set iff the value is above the median
+// pivot of column k (pivot 3 of the 7 quantile pivots).  If q dominates p
+// (even weakly) then q_k <= p_k, so mask(q) is a subset of mask(p).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_self(const float* __restrict__ rows,
-                                                 const uint32_t* __restrict__ keys, uint32_t m,
-                                                 int need_dom, uint8_t* __restrict__ status) {
-  __shared__ float tile[kBlock * D];
-  __shared__ uint32_t tkey[kBlock];
-  __shared__ int s_active;
-  constexpr uint32_t full = (D == 32) ? 0xFFFFFFFFu : ((1u << D) - 1u);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = i < m;
-  float p[D];
-  uint32_t kp = 0;
-  if (valid) {
+__global__ void k_median_masks(const float* __restrict__ rows, uint32_t m,
+                               const uint32_t* __restrict__ piv, uint32_t* __restrict__ masks) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t w = 0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
-    kp = keys[i];
-  }
-  const uint32_t last = min(m - 1u, blockIdx.x * kBlock + kBlock - 1u);
-  const uint32_t kmax = keys[last];
+  for (int k = 0; k < D; ++k) w |= (uint32_t)(order_key(rows[(size_t)i * D + k]) > piv[k * 7 + 3]) << k;
+  masks[i] = w;
+}
+
+// Survivors against each other, one wave64 per point p: status = 2 if
+// strictly dominated, else 1 if dominated (only when need_dom), else 0.
+// The lanes walk the candidates q with key <= key(p) (score order) 64 at a
+// time; q can (weakly) dominate p only if mask(q) is a subset of mask(p),
+// so only those are kept -- compacted with a ballot into a per-wave LDS
+// queue -- and get the exact FP32 test once 64 have gathered.  The wave
+// stops at the first strict dominator.  One SkyAlign pass
+// (total_pruning_moved_data_templated_pitch_two_level_full_warps,
+// src/utilities/kernels.cuh:36-411) over the whole candidate set, with its
+// partition containment test (other subset of mine) as the filter.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ rows,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ masks,
+                                                      uint32_t m, int need_dom,
+                                                      uint8_t* __restrict__ status) {
+  constexpr int WAVES = kBlock / 64;
+  constexpr uint32_t FULL = (1u << D) - 1u;
+  __shared__ uint32_t qbuf[WAVES][128];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * WAVES + wv;
+  if (i >= m) return;  // whole wave; the kernel has no block barrier
+  float p[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
+  const uint32_t kp = keys[i];
+  const uint32_t mp = masks[i];
+  uint32_t* q = qbuf[wv];
+  int qn = 0;
   uint8_t st = 0;
-  bool done = !valid;
-  for (uint32_t t0 = 0; t0 < m; t0 += kBlock) {
-    if (keys[t0] > kmax) break;  // uniform: keys ascend
-    const uint32_t cnt = min((uint32_t)kBlock, m - t0);
-    __syncthreads();
-    if (threadIdx.x == 0) s_active = 0;
-    for (uint32_t e = threadIdx.x; e < cnt * D; e += kBlock) tile[e] = rows[(size_t)t0 * D + e];
-    if (threadIdx.x < cnt) tkey[threadIdx.x] = keys[t0 + threadIdx.x];
-    __syncthreads();
-    if (!done) {
-      for (uint32_t t = 0; t < cnt; ++t) {
-        if (tkey[t] > kp) {
-          done = true;
-          break;
-        }
-        if (t0 + t == i) continue;
+  for (uint32_t b0 = 0; b0 < m; b0 += 64) {
+    const uint32_t t = b0 + lane;
+    const bool in = t < m && keys[t] <= kp;
+    const bool cand = in && t != i && (masks[t] & ~mp) == 0;
+    const unsigned long long bal = __ballot(cand);
+    if (cand) q[qn + __popcll(bal & ((1ull << lane) - 1ull))] = t;
+    qn += __popcll(bal);
+    __builtin_amdgcn_wave_barrier();
+    // more candidates may follow only if some lane was in range and the
+    // array goes on; otherwise the queue is drained now
+    const bool more = __ballot(in) != 0ull && b0 + 64 < m;
+    if (qn >= 64 || (!more && qn > 0)) {
+      bool strict = false, dom = false;
+      for (int e = lane; e < qn; e += 64) {
+        const float* qr = rows + (size_t)q[e] * D;
         uint32_t lt = 0, eq = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          const float q = tile[t * D + k];
-          lt |= (uint32_t)(q < p[k]) << k;
-          eq |= (uint32_t)(q == p[k]) << k;
+          const float v = qr[k];
+          lt |= (uint32_t)(v < p[k]) << k;
+          eq |= (uint32_t)(v == p[k]) << k;
         }
-        if (lt == full) {
-          st = 2;
-          done = true;
-          break;
-        }
-        if (need_dom && lt != 0 && (lt | eq) == full) st = 1;
+        strict |= lt == FULL;
+        dom |= lt != 0 && (lt | eq) == FULL;
       }
+      qn = 0;
+      if (__ballot(strict)) {
+        st = 2;
+        break;
+      }
+      if (need_dom && __ballot(dom)) st = 1;
     }
-    if (!done) s_active = 1;
-    __syncthreads();
-    if (!s_active) break;
+    if (!more) break;
   }
-  if (valid) status[i] = st;
+  if (lane == 0) status[i] = st;
 }
 
 // flag[pid[j]] = 1 for every survivor j that nothing strictly dominates.

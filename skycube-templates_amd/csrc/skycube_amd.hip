@@ -37,7 +37,8 @@ struct sky_ctx {
   std::string err;
   // device buffers
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
-      status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv;
+      status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
+      masks;
   // stage timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
@@ -201,6 +202,8 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
   TRY(ensure(c, c->s1keys, sizeof(uint32_t) * nc));
   TRY(ensure(c, c->s1pid, sizeof(uint32_t) * nc));
   TRY(ensure(c, c->status, nc));
+  TRY(ensure(c, c->masks, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->piv, sizeof(uint32_t) * 16 * sky::kCodePivots));
   TRY(ensure(c, c->counter, 64 * sizeof(uint32_t)));
 
   TRY(mark(c, SKY_STAGE_SCORE_SORT));
@@ -246,9 +249,15 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
                        P<uint32_t>(c->s1pid));
     LAUNCHCHK();
     TRY(mark(c, SKY_STAGE_SELF));
-    hipLaunchKernelGGL(sky::k_self<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
-                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), m, need_dom,
-                       P<uint8_t>(c->status));
+    hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), m, P<uint32_t>(c->piv));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_median_masks<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), m, P<uint32_t>(c->piv), P<uint32_t>(c->masks));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_self_wave<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), P<uint32_t>(c->masks), m,
+                       need_dom, P<uint8_t>(c->status));
     LAUNCHCHK();
   });
   *m_out = m;
@@ -414,7 +423,7 @@ void sky_ctx_destroy(sky_ctx* c) {
                     &c->srows,  &c->alive,  &c->s1idx,    &c->s1rows, &c->s1keys,
                     &c->s1pid,  &c->status, &c->flag,     &c->ext_pids, &c->erows,
                     &c->dom,    &c->cub_tmp, &c->counter, &c->cand,  &c->eidx,
-                    &c->epid,   &c->pos,    &c->codes,    &c->piv};
+                    &c->epid,   &c->pos,    &c->codes,    &c->piv,   &c->masks};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : c->ev)
