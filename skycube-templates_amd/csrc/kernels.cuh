@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ 
                                                       uint8_t* __restrict__ status) {
   constexpr int WAVES = kBlock / 64;
   constexpr uint32_t FULL = (1u << D) - 1u;
-  __shared__ uint32_t qbuf[WAVES][128];
+  __shared__ uint32_t qbuf[WAVES][192];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * WAVES + wv;
@@ -173,21 +173,33 @@ __global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ 
   for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
   const uint32_t kp = keys[i];
   const uint32_t mp = masks[i];
+  // ub = first position with key > kp (keys ascend)
+  uint32_t lo = i + 1, hi = m;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] <= kp) lo = mid + 1;
+    else hi = mid;
+  }
+  const uint32_t ub = lo;
   uint32_t* q = qbuf[wv];
   int qn = 0;
   uint8_t st = 0;
-  for (uint32_t b0 = 0; b0 < m; b0 += 64) {
-    const uint32_t t = b0 + lane;
-    const bool in = t < m && keys[t] <= kp;
-    const bool cand = in && t != i && (masks[t] & ~mp) == 0;
-    const unsigned long long bal = __ballot(cand);
-    if (cand) q[qn + __popcll(bal & ((1ull << lane) - 1ull))] = t;
-    qn += __popcll(bal);
+  for (uint32_t b0 = 0; b0 < ub; b0 += 128) {
+    const uint32_t t0 = b0 + lane, t1 = t0 + 64;
+    const uint32_t m0 = t0 < ub ? masks[t0] : 0xFFFFFFFFu;
+    const uint32_t m1 = t1 < ub ? masks[t1] : 0xFFFFFFFFu;
+    const bool c0 = t0 < ub && t0 != i && (m0 & ~mp) == 0;
+    const bool c1 = t1 < ub && t1 != i && (m1 & ~mp) == 0;
+    const unsigned long long b0l = __ballot(c0);
+    const unsigned long long b1l = __ballot(c1);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (c0) q[qn + __popcll(b0l & below)] = t0;
+    qn += __popcll(b0l);
+    if (c1) q[qn + __popcll(b1l & below)] = t1;
+    qn += __popcll(b1l);
     __builtin_amdgcn_wave_barrier();
-    // more candidates may follow only if some lane was in range and the
-    // array goes on; otherwise the queue is drained now
-    const bool more = __ballot(in) != 0ull && b0 + 64 < m;
-    if (qn >= 64 || (!more && qn > 0)) {
+    const bool last = b0 + 128 >= ub;
+    if (qn >= 64 || (last && qn > 0)) {
       bool strict = false, dom = false;
       for (int e = lane; e < qn; e += 64) {
         const float* qr = rows + (size_t)q[e] * D;
@@ -208,7 +220,6 @@ __global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ 
       }
       if (need_dom && __ballot(dom)) st = 1;
     }
-    if (!more) break;
   }
   if (lane == 0) status[i] = st;
 }
@@ -315,7 +326,11 @@ __device__ __forceinline__ void lds_mark(uint32_t* w, uint32_t s) {
   if (!(w[s >> 5] & bit)) atomicOr(&w[s >> 5], bit);
 }
 
-constexpr int kZMax = 10;     // boundary columns tracked per point (2^10 words)
+// Boundary columns tracked per point (a 2^ZB-word table per point).
+template <int D>
+constexpr int zbits() {
+  return D < 8 ? D : (D >= 15 ? 10 : 8);
+}
 constexpr int kCodePivots = 7;  // 8 quantile buckets per column, 3-bit codes
 constexpr uint32_t kCover = 2048;  // lowest-score candidates scanned first
 
@@ -353,13 +368,16 @@ __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ ero
 
 // 3-bit quantile code of every column, 8 columns per word (nibbles); the
 // code is the number of pivots below the value, so v <= w implies
-// code(v) <= code(w).
+// code(v) <= code(w).  masks[i] holds the codes' top bits (value above the
+// median pivot), a coarser code with the same property.
 template <int D>
 __global__ void k_codes(const float* __restrict__ erows, uint32_t m,
-                        const uint32_t* __restrict__ piv, uint2* __restrict__ codes) {
+                        const uint32_t* __restrict__ piv, uint2* __restrict__ codes,
+                        uint32_t* __restrict__ masks) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   uint32_t w[2] = {0u, 0u};
+  uint32_t med = 0;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     const uint32_t key = order_key(erows[(size_t)i * D + k]);
@@ -367,8 +385,10 @@ __global__ void k_codes(const float* __restrict__ erows, uint32_t m,
 #pragma unroll
     for (int b = 0; b < kCodePivots; ++b) c += piv[k * kCodePivots + b] < key;
     w[k >> 3] |= c << ((k & 7) * 4);
+    med |= (c >> 2) << k;
   }
   codes[i] = make_uint2(w[0], w[1]);
+  masks[i] = med;
 }
 
 // Bit k (k < 8) is set iff nibble k of q is <= nibble k of p (3-bit values):
@@ -448,7 +468,7 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 //
 // For a point p and a candidate q: lt = {i: q_i < p_i}, eq = {i: q_i ==
 // p_i}, L = lt|eq.  q dominates p in s iff s is inside L and s meets lt.
-// Boundary columns Z = {j: p_j is the column minimum} (at most kZMax of them)
+// Boundary columns Z = {j: p_j is the column minimum} (at most zbits<D>())
 // admit no q_j < p_j, so for j in Z, j in L iff j in eq.  Write s = sZ + sR
 // (sR outside Z) and eR = eq outside Z (the residual ties):
 //   sR empty          -> never dominated (no strict column inside s)
@@ -468,17 +488,19 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // residual tie, which dominates p in every subspace (sR non-empty) that a
 // pair with a smaller L could, and carries its U contribution too.  So a
 // later q whose code bound on L (columns with code_q <= code_p, a superset
-// of L) is set in A0 adds nothing and is skipped.  Survivors of the code
+// of L; first from the median bits, then from the 3-bit codes) is set in A0
+// adds nothing and is skipped.  Survivors of the code
 // test are compacted into an LDS queue and get the exact FP32 test together.
 // Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
 template <int D, int P>
 __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
                                                    const uint2* __restrict__ codes,
+                                                   const uint32_t* __restrict__ meds,
                                                    const uint32_t* __restrict__ pos, uint32_t m,
                                                    int max_d, const uint32_t* __restrict__ minkey,
                                                    uint32_t* __restrict__ dom) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
-  constexpr int ZB = D < kZMax ? D : kZMax;
+  constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
   constexpr int QCAP = kBlock * (P + 1);
   constexpr uint32_t FULL = (1u << D) - 1u;
@@ -544,14 +566,22 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
 
   // pass 1b: code test, queue, exact test of the survivors
   uint2 pc[P];
+  uint32_t pm[P];
 #pragma unroll
-  for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
+  for (int pp = 0; pp < P; ++pp) {
+    pc[pp] = codes[min(base + pp, m - 1u)];
+    pm[pp] = meds[min(base + pp, m - 1u)];
+  }
   for (uint32_t c0 = kc; c0 < m; c0 += kBlock) {
     const uint32_t b = c0 + tid;
     if (b < m) {
+      const uint32_t qm = meds[b];
       const uint2 qc = codes[b];
 #pragma unroll
       for (int pp = 0; pp < P; ++pp) {
+        // median-mask bound first, the 3-bit code bound only if it fails
+        const uint32_t ub1 = ~(qm & ~pm[pp]) & FULL;
+        if ((A0[pp][ub1 >> 5] >> (ub1 & 31)) & 1u) continue;
         uint32_t ub = code_le(qc.x, pc[pp].x);
         if (D > 8) ub |= code_le(qc.y, pc[pp].y) << 8;
         ub &= FULL;
