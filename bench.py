@@ -89,6 +89,8 @@ def main() -> int:
     ap.add_argument("--workload", default="integer_ties")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="sha256 of the result vs the golden digest")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every run here uses a single MI355X "
@@ -97,6 +99,16 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(max(1, args.steps)):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": max(1, args.steps)}))
+        return 0
 
     import skycube_amd
     from skycube_amd import datagen

@@ -116,6 +116,24 @@ int sky_mdmc_device_results(sky_ctx *ctx, const uint32_t **ext_pids_dev,
                             const uint32_t **dom_words_dev);
 
 /*
+ * HashCube of the last MDMC result: the reference's result structure
+ * (HashCube::populateFromSingleNodeAux, src/skycube/Hashcube.h:107-115, built
+ * by MDMC::build_hashcube, src/mdmc/MDMC.cu:648-661) -- for every word index
+ * j, a map from word value v to the ids of the points whose bitmap word j is
+ * v, leaving out values with all 32 bits set.  Like the reference's NodeAux
+ * bitmap, a point has sky_words_per_point(d) + 1 words when d >= 5, the last
+ * one always 0.  Grouped on the device (sort by (j, v)); returned as groups
+ * ascending by (j, v): group_word[g], group_value[g], and the ids
+ * pids[group_off[g] .. group_off[g+1]) ascending.  Call with NULL arrays (or
+ * too small capacities) to learn *n_groups and *n_entries first; the grouping
+ * is kept on the device until the next call.
+ */
+int sky_mdmc_hashcube(sky_ctx *ctx, uint64_t *n_groups, uint64_t *n_entries,
+                      uint32_t *group_word, uint32_t *group_value,
+                      uint64_t *group_off, uint32_t *pids, uint64_t cap_groups,
+                      uint64_t cap_entries);
+
+/*
  * One-cuboid SkyAlign: sky(s) and ext(s) \ sky(s) over a candidate set.
  * Replaces run_skyalign(full_dims, n, dimension_set, sky, ext, working_set,
  * conf) (src/sdsc/skyAlign.cu:9-53, src/sdsc/skyAlign.cuh:321-322).

@@ -735,4 +735,50 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
   }
 }
 
+// ---------------------------------------------------------------------------
+// HashCube export: (word index, word value) -> point ids.
+
+// Words of point i that the HashCube keeps (not all ones), plus the extra
+// always-zero word for d >= 5.
+__global__ void k_hc_count(const uint32_t* __restrict__ dom, uint32_t ne, uint32_t W,
+                           uint32_t extra, uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  uint32_t c = extra;
+  for (uint32_t j = 0; j < W; ++j) c += dom[(size_t)i * W + j] != 0xFFFFFFFFu;
+  cnt[i] = c;
+}
+
+// Entries (key = j << 32 | value, id) of point i from offset off[i].
+__global__ void k_hc_emit(const uint32_t* __restrict__ dom, const uint32_t* __restrict__ ids,
+                          uint32_t ne, uint32_t W, uint32_t extra,
+                          const uint64_t* __restrict__ off, uint64_t* __restrict__ key,
+                          uint32_t* __restrict__ val) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  uint64_t o = off[i];
+  const uint32_t pid = ids[i];
+  for (uint32_t j = 0; j < W; ++j) {
+    const uint32_t v = dom[(size_t)i * W + j];
+    if (v != 0xFFFFFFFFu) {
+      key[o] = ((uint64_t)j << 32) | v;
+      val[o] = pid;
+      ++o;
+    }
+  }
+  if (extra) {
+    key[o] = (uint64_t)W << 32;
+    val[o] = pid;
+  }
+}
+
+// Splits run keys into (word, value) and run lengths into 64-bit offsets.
+__global__ void k_hc_groups(const uint64_t* __restrict__ ukey, uint64_t ng,
+                            uint32_t* __restrict__ gw, uint32_t* __restrict__ gv) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  gw[g] = (uint32_t)(ukey[g] >> 32);
+  gv[g] = (uint32_t)ukey[g];
+}
+
 }  // namespace sky

@@ -38,7 +38,11 @@ struct sky_ctx {
   // device buffers
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
-      masks;
+      masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
+      hc_gv, hc_goff;
+  // grouping of the last HashCube export
+  bool has_hc = false;
+  uint64_t hc_groups = 0, hc_entries = 0;
   // stage timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
@@ -274,6 +278,7 @@ int check_ctx(sky_ctx* c) {
 int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
              uint32_t* n_ext) {
   c->has_mdmc = false;
+  c->has_hc = false;
   const uint32_t W = sky_words_per_point(d);
   if (max_d == 0 || max_d > d) max_d = d;
   *n_ext = 0;
@@ -393,6 +398,110 @@ int skyalign_dev(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d,
   return SKY_OK;
 }
 
+// Groups the last MDMC bitmaps into HashCube entries on the device.
+int build_hashcube(sky_ctx* c) {
+  const uint32_t ne = c->res_n_ext, W = c->res_W;
+  const uint32_t extra = c->res_d >= 5 ? 1u : 0u;
+  c->hc_groups = c->hc_entries = 0;
+  if (ne == 0) {
+    c->has_hc = true;
+    return SKY_OK;
+  }
+  TRY(ensure(c, c->hc_cnt, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->hc_off, sizeof(uint64_t) * (ne + 1)));
+  hipLaunchKernelGGL(sky::k_hc_count, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->dom), ne, W, extra, P<uint32_t>(c->hc_cnt));
+  LAUNCHCHK();
+  // 64-bit exclusive scan of the counts (ne + 1 entries: the last is the total)
+  TRY(ensure(c, c->hc_runs, sizeof(uint64_t) * (ne + 1)));
+  uint64_t* cnt64 = P<uint64_t>(c->hc_runs);
+  HIPCHK(hipMemsetAsync(cnt64 + ne, 0, sizeof(uint64_t), c->stream));
+  {
+    hipcub::TransformInputIterator<uint64_t, hipcub::CastOp<uint64_t>, const uint32_t*> it(
+        P<uint32_t>(c->hc_cnt), hipcub::CastOp<uint64_t>());
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, P<uint64_t>(c->hc_off), ne,
+                                            c->stream));
+    TRY(ensure(c, c->cub_tmp, bytes));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, it, P<uint64_t>(c->hc_off), ne,
+                                            c->stream));
+  }
+  uint64_t last_off = 0;
+  uint32_t last_cnt = 0;
+  HIPCHK(hipMemcpyAsync(&last_off, P<uint64_t>(c->hc_off) + (ne - 1), sizeof(uint64_t),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&last_cnt, P<uint32_t>(c->hc_cnt) + (ne - 1), sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const uint64_t total = last_off + last_cnt;
+  if (total > 0xFFFFFFFFull)
+    return fail(c, SKY_ERR_CAPACITY, "HashCube export of %llu entries exceeds 2^32",
+                (unsigned long long)total);
+  TRY(ensure(c, c->hc_key_a, sizeof(uint64_t) * total));
+  TRY(ensure(c, c->hc_key_b, sizeof(uint64_t) * total));
+  TRY(ensure(c, c->hc_val_a, sizeof(uint32_t) * total));
+  TRY(ensure(c, c->hc_val_b, sizeof(uint32_t) * total));
+  hipLaunchKernelGGL(sky::k_hc_emit, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->dom), P<uint32_t>(c->ext_pids), ne, W, extra,
+                     P<uint64_t>(c->hc_off), P<uint64_t>(c->hc_key_a), P<uint32_t>(c->hc_val_a));
+  LAUNCHCHK();
+  int end_bit = 32;
+  while ((1ull << (end_bit - 32)) <= (uint64_t)W) ++end_bit;  // word index bits
+  {
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P<uint64_t>(c->hc_key_a),
+                                              P<uint64_t>(c->hc_key_b), P<uint32_t>(c->hc_val_a),
+                                              P<uint32_t>(c->hc_val_b), (uint32_t)total, 0,
+                                              end_bit, c->stream));
+    TRY(ensure(c, c->cub_tmp, bytes));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, P<uint64_t>(c->hc_key_a),
+                                              P<uint64_t>(c->hc_key_b), P<uint32_t>(c->hc_val_a),
+                                              P<uint32_t>(c->hc_val_b), (uint32_t)total, 0,
+                                              end_bit, c->stream));
+  }
+  // runs of equal keys = groups
+  TRY(ensure(c, c->hc_ukey, sizeof(uint64_t) * total));
+  TRY(ensure(c, c->hc_gw, sizeof(uint32_t) * total));
+  TRY(ensure(c, c->hc_goff, sizeof(uint64_t) * (total + 1)));
+  uint32_t* run_len = P<uint32_t>(c->hc_val_a);  // reuse: the unsorted values are done
+  {
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, P<uint64_t>(c->hc_key_b),
+                                                 P<uint64_t>(c->hc_ukey), run_len,
+                                                 P<uint32_t>(c->counter), (uint32_t)total,
+                                                 c->stream));
+    TRY(ensure(c, c->cub_tmp, bytes));
+    HIPCHK(hipcub::DeviceRunLengthEncode::Encode(c->cub_tmp.p, bytes, P<uint64_t>(c->hc_key_b),
+                                                 P<uint64_t>(c->hc_ukey), run_len,
+                                                 P<uint32_t>(c->counter), (uint32_t)total,
+                                                 c->stream));
+  }
+  uint32_t ng = 0;
+  TRY(copy_count(c, &ng));
+  TRY(ensure(c, c->hc_gv, sizeof(uint32_t) * (ng + 1)));
+  {
+    hipcub::TransformInputIterator<uint64_t, hipcub::CastOp<uint64_t>, const uint32_t*> it(
+        run_len, hipcub::CastOp<uint64_t>());
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, P<uint64_t>(c->hc_goff), ng,
+                                            c->stream));
+    TRY(ensure(c, c->cub_tmp, bytes));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, it, P<uint64_t>(c->hc_goff), ng,
+                                            c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(P<uint64_t>(c->hc_goff) + ng, &total, sizeof(uint64_t),
+                        hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(sky::k_hc_groups, dim3(grid_for(ng)), dim3(kBlock), 0, c->stream,
+                     P<uint64_t>(c->hc_ukey), (uint64_t)ng, P<uint32_t>(c->hc_gw),
+                     P<uint32_t>(c->hc_gv));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->hc_groups = ng;
+  c->hc_entries = total;
+  c->has_hc = true;
+  return SKY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -425,7 +534,10 @@ void sky_ctx_destroy(sky_ctx* c) {
                     &c->srows,  &c->alive,  &c->s1idx,    &c->s1rows, &c->s1keys,
                     &c->s1pid,  &c->status, &c->flag,     &c->ext_pids, &c->erows,
                     &c->dom,    &c->cub_tmp, &c->counter, &c->cand,  &c->eidx,
-                    &c->epid,   &c->pos,    &c->codes,    &c->piv,   &c->masks};
+                    &c->epid,   &c->pos,    &c->codes,    &c->piv,   &c->masks,
+                    &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
+                    &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
+                    &c->hc_goff};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : c->ev)
@@ -497,6 +609,36 @@ int sky_mdmc_device_results(sky_ctx* c, const uint32_t** ext_pids_dev,
   if (!c->has_mdmc) return fail(c, SKY_ERR_STATE, "no MDMC result is held");
   if (ext_pids_dev) *ext_pids_dev = P<uint32_t>(c->ext_pids);
   if (dom_words_dev) *dom_words_dev = P<uint32_t>(c->dom);
+  return SKY_OK;
+}
+
+int sky_mdmc_hashcube(sky_ctx* c, uint64_t* n_groups, uint64_t* n_entries, uint32_t* group_word,
+                      uint32_t* group_value, uint64_t* group_off, uint32_t* pids,
+                      uint64_t cap_groups, uint64_t cap_entries) {
+  TRY(check_ctx(c));
+  if (!n_groups || !n_entries) return fail(c, SKY_ERR_ARG, "null argument");
+  if (!c->has_mdmc) return fail(c, SKY_ERR_STATE, "no MDMC result is held");
+  if (!c->has_hc) TRY(build_hashcube(c));
+  *n_groups = c->hc_groups;
+  *n_entries = c->hc_entries;
+  if (!group_word || !group_value || !group_off || !pids) return SKY_OK;
+  if (cap_groups < c->hc_groups || cap_entries < c->hc_entries)
+    return fail(c, SKY_ERR_CAPACITY, "HashCube needs %llu groups and %llu entries",
+                (unsigned long long)c->hc_groups, (unsigned long long)c->hc_entries);
+  const uint64_t ng = c->hc_groups, nt = c->hc_entries;
+  if (ng) {
+    HIPCHK(hipMemcpyAsync(group_word, c->hc_gw.p, sizeof(uint32_t) * ng, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(group_value, c->hc_gv.p, sizeof(uint32_t) * ng,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(group_off, c->hc_goff.p, sizeof(uint64_t) * (ng + 1),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pids, c->hc_val_b.p, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost,
+                          c->stream));
+  } else {
+    group_off[0] = 0;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return SKY_OK;
 }
 

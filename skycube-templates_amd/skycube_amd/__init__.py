@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "SkyError", "Context", "library_path", "load", "words_per_point",
-    "in_scope_mask", "members", "SKY_OK",
+    "in_scope_mask", "members", "SKY_OK", "hashcube_query", "hashcube_from_bitmaps",
 ]
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -35,7 +35,7 @@ EXPORTS = (
     "sky_ctx_create", "sky_ctx_destroy", "sky_last_error", "sky_words_per_point",
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
-    "sky_ctx_stage_times",
+    "sky_ctx_stage_times", "sky_mdmc_hashcube",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -87,6 +87,8 @@ def load():
     L.sky_sdsc.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                            ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p,
                            ctypes.c_uint64]
+    L.sky_mdmc_hashcube.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u32p, _u32p, _u64p, _u32p,
+                                     ctypes.c_uint64, ctypes.c_uint64]
     L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     _lib = L
@@ -118,6 +120,37 @@ def members(dom: np.ndarray, ext_pids: np.ndarray, s: int) -> np.ndarray:
     is clear (the query of getsubspace, src/utilities/utilities.h:253-267)."""
     w, b = divmod(int(s), 32)
     return np.asarray(ext_pids)[((dom[:, w] >> np.uint32(b)) & 1) == 0]
+
+
+def hashcube_query(group_word, group_value, group_off, pids, s: int) -> np.ndarray:
+    """Skyline of cuboid s from a HashCube: the ids of every group of word
+    s/32 whose value has bit s%32 clear (getsubspace,
+    src/utilities/utilities.h:253-267), ascending."""
+    w, b = divmod(int(s), 32)
+    sel = np.flatnonzero((group_word == w) & (((group_value >> np.uint32(b)) & 1) == 0))
+    parts = [pids[group_off[g]:group_off[g + 1]] for g in sel]
+    return np.sort(np.concatenate(parts)) if parts else np.zeros(0, dtype=np.uint32)
+
+
+def hashcube_from_bitmaps(ext_pids, dom, d: int):
+    """Host grouping of membership bitmaps into HashCube groups (the same
+    layout as Context.mdmc_hashcube), for checks."""
+    W = dom.shape[1] if dom.ndim == 2 else 1
+    words = [dom[:, j] for j in range(W)]
+    if d >= 5:
+        words.append(np.zeros(len(ext_pids), dtype=np.uint32))
+    keys, ids = [], []
+    for j, col in enumerate(words):
+        keep = col != 0xFFFFFFFF
+        keys.append((np.uint64(j) << np.uint64(32)) | col[keep].astype(np.uint64))
+        ids.append(np.asarray(ext_pids)[keep])
+    keys = np.concatenate(keys) if keys else np.zeros(0, np.uint64)
+    ids = np.concatenate(ids) if ids else np.zeros(0, np.uint32)
+    order = np.lexsort((ids, keys))
+    keys, ids = keys[order], ids[order]
+    uk, start = np.unique(keys, return_index=True)
+    off = np.append(start, keys.size).astype(np.uint64)
+    return (uk >> np.uint64(32)).astype(np.uint32), (uk & np.uint64(0xFFFFFFFF)).astype(np.uint32), off, ids.astype(np.uint32)
 
 
 def _ptr(a: np.ndarray, t):
@@ -214,6 +247,23 @@ class Context:
         w = ctypes.c_void_p()
         self._check(self._L.sky_mdmc_device_results(self._h, ctypes.byref(e), ctypes.byref(w)))
         return e.value, w.value
+
+    def mdmc_hashcube(self):
+        """HashCube of the last MDMC result, grouped on the device:
+        (group_word, group_value, group_off [n_groups+1], pids)."""
+        ng = np.zeros(1, dtype=np.uint64)
+        nt = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.sky_mdmc_hashcube(self._h, _ptr(ng, _u64p), _ptr(nt, _u64p), None,
+                                              None, None, None, 0, 0))
+        g, t = int(ng[0]), int(nt[0])
+        gw = np.zeros(max(g, 1), dtype=np.uint32)
+        gv = np.zeros(max(g, 1), dtype=np.uint32)
+        go = np.zeros(g + 1, dtype=np.uint64)
+        ids = np.zeros(max(t, 1), dtype=np.uint32)
+        self._check(self._L.sky_mdmc_hashcube(self._h, _ptr(ng, _u64p), _ptr(nt, _u64p),
+                                              _ptr(gw, _u32p), _ptr(gv, _u32p), _ptr(go, _u64p),
+                                              _ptr(ids, _u32p), max(g, 1), max(t, 1)))
+        return gw[:g], gv[:g], go, ids[:t]
 
     def skyalign(self, rows, subspace: int, cand=None):
         """(sky ids ascending, ext-but-not-sky ids ascending) of cuboid `subspace`."""

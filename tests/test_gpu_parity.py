@@ -152,3 +152,20 @@ def test_sdsc_matches_stsc(ctx, n, d, max_d):
         got_ext = ext[eo[s]:eo[s + 1]]
         assert np.array_equal(got_sky, np.flatnonzero(~D[:, s])), s
         assert np.array_equal(got_ext, np.flatnonzero(D[:, s] & ~SD[:, s])), s
+
+
+@pytest.mark.parametrize("d,max_d,n", [(3, 3, 500), (6, 6, 2000), (9, 4, 3000), (12, 12, 1500)])
+def test_hashcube_export(ctx, d, max_d, n):
+    """Device-grouped HashCube equals the host grouping of the bitmaps, and
+    its query gives every cuboid skyline."""
+    x = datagen.integer_ties(n, d, 7000 + d)
+    _say(f"hashcube d={d} max_d={max_d} n={n}")
+    ext, dom = ctx.mdmc(x, max_d)
+    gw, gv, go, ids = ctx.mdmc_hashcube()
+    hw, hv, ho, hids = skycube_amd.hashcube_from_bitmaps(ext, dom, d)
+    assert np.array_equal(gw, hw) and np.array_equal(gv, hv)
+    assert np.array_equal(go, ho) and np.array_equal(ids, hids)
+    scope = npref.scope(d, max_d)
+    for s in np.flatnonzero(scope)[:200]:
+        assert np.array_equal(skycube_amd.hashcube_query(gw, gv, go, ids, s),
+                              np.sort(skycube_amd.members(dom, ext, s)))
