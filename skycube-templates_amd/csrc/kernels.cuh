@@ -19,6 +19,7 @@ namespace sky {
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 constexpr uint32_t kPruners = 1024;  // lowest-score points used as a prefilter
 constexpr int kXCap = 2048;       // tie-touched subspaces resolved per round
+constexpr int kCodePivots = 15;   // 16 quantile buckets per column, 4-bit codes
 
 struct DimList {
   int32_t idx[16];
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_prune(const float* __restrict__ srow
 }
 
 // Median mask of each row: bit k set iff the value is above the median
-// pivot of column k (pivot 3 of the 7 quantile pivots).  If q dominates p
+// pivot of column k (pivot 7 of the 15 quantile pivots).  If q dominates p
 // (even weakly) then q_k <= p_k, so mask(q) is a subset of mask(p).
 template <int D>
 __global__ void k_median_masks(const float* __restrict__ rows, uint32_t m,
@@ -141,7 +142,7 @@ __global__ void k_median_masks(const float* __restrict__ rows, uint32_t m,
   if (i >= m) return;
   uint32_t w = 0;
 #pragma unroll
-  for (int k = 0; k < D; ++k) w |= (uint32_t)(order_key(rows[(size_t)i * D + k]) > piv[k * 7 + 3]) << k;
+  for (int k = 0; k < D; ++k) w |= (uint32_t)(order_key(rows[(size_t)i * D + k]) > piv[k * kCodePivots + 7]) << k;
   masks[i] = w;
 }
 
@@ -331,7 +332,6 @@ template <int D>
 constexpr int zbits() {
   return D < 8 ? D : (D >= 15 ? 10 : 8);
 }
-constexpr int kCodePivots = 7;  // 8 quantile buckets per column, 3-bit codes
 constexpr uint32_t kCover = 2048;  // lowest-score candidates scanned first
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB.
@@ -340,10 +340,10 @@ constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
 }
 
-// Quantile pivots of each column of the extended skyline, as order keys:
-// block = one column; a strided sample of up to 2048 values is ranked by
-// counting and the values at ranks b*S/8 (b = 1..7) become the pivots.  Any
-// pivots give correct codes; good ones make the codes selective.
+// Quantile pivots of each column, as order keys: block = one column; a
+// strided sample of up to 2048 values is ranked by counting and the values
+// at ranks b*S/16 (b = 1..15) become the pivots.  Any pivots give correct
+// codes; good ones make the codes selective.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ erows, uint32_t m,
                                                    uint32_t* __restrict__ piv) {
@@ -366,40 +366,41 @@ __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ ero
     piv[col * kCodePivots + threadIdx.x] = srt[(threadIdx.x + 1) * cnt / (kCodePivots + 1)];
 }
 
-// 3-bit quantile code of every column, 8 columns per word (nibbles); the
-// code is the number of pivots below the value, so v <= w implies
-// code(v) <= code(w).  masks[i] holds the codes' top bits (value above the
-// median pivot), a coarser code with the same property.
+// 4-bit quantile code of every column (the number of pivots below the
+// value, so v <= w implies code(v) <= code(w)), stored as bit planes: bit k
+// of plane b is bit b of column k's code.  x = plane3 << 16 | plane1,
+// y = plane2 << 16 | plane0.  masks[i] = plane 3: the column is above the
+// median pivot.
 template <int D>
 __global__ void k_codes(const float* __restrict__ erows, uint32_t m,
                         const uint32_t* __restrict__ piv, uint2* __restrict__ codes,
                         uint32_t* __restrict__ masks) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  uint32_t w[2] = {0u, 0u};
-  uint32_t med = 0;
+  uint32_t pl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     const uint32_t key = order_key(erows[(size_t)i * D + k]);
     uint32_t c = 0;
 #pragma unroll
     for (int b = 0; b < kCodePivots; ++b) c += piv[k * kCodePivots + b] < key;
-    w[k >> 3] |= c << ((k & 7) * 4);
-    med |= (c >> 2) << k;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pl[b] |= ((c >> b) & 1u) << k;
   }
-  codes[i] = make_uint2(w[0], w[1]);
-  masks[i] = med;
+  codes[i] = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+  masks[i] = pl[3];
 }
 
-// Bit k (k < 8) is set iff nibble k of q is <= nibble k of p (3-bit values):
-// the guard bit of each nibble of (p | 8..8) - q survives iff p >= q; the
-// eight guard bits are then folded into one byte.
-__device__ __forceinline__ uint32_t code_le(uint32_t q, uint32_t p) {
-  const uint32_t x = (p | 0x88888888u) - q;
-  uint32_t g = (x >> 3) & 0x11111111u;
-  g = (g | (g >> 3)) & 0x03030303u;
-  g = (g | (g >> 6)) & 0x000F000Fu;
-  return (g | (g >> 12)) & 0xFFu;
+// Columns k where code_q(k) <= code_p(k), for the bit-plane codes above
+// (a lexicographic compare of the planes, 16 columns at a time): the upper
+// halves compare planes 3:2, the lower halves planes 1:0, and the upper
+// result decides unless the upper planes are equal.
+__device__ __forceinline__ uint32_t planes_le(uint2 q, uint2 p) {
+  const uint32_t le_b = ~q.y | p.y;                       // plane 2 | plane 0
+  const uint32_t le2 = (~q.x & p.x) | (~(q.x ^ p.x) & le_b);  // planes 3:2 | 1:0
+  const uint32_t ne_hi = ((q.x ^ p.x) | (q.y ^ p.y)) >> 16;   // planes 3:2 differ
+  const uint32_t hi = le2 >> 16, lo = le2 & 0xFFFFu;
+  return (ne_hi & hi) | (~ne_hi & lo);
 }
 
 // Marks one (q, p) pair with masks lt, eq into the structures of point pp.
@@ -488,21 +489,20 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // residual tie, which dominates p in every subspace (sR non-empty) that a
 // pair with a smaller L could, and carries its U contribution too.  So a
 // later q whose code bound on L (columns with code_q <= code_p, a superset
-// of L; first from the median bits, then from the 3-bit codes) is set in A0
-// adds nothing and is skipped.  Survivors of the code
+// of L, from the 4-bit bit-plane codes) is set in A0 adds nothing and is
+// skipped.  Survivors of the code
 // test are compacted into an LDS queue and get the exact FP32 test together.
 // Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
 template <int D, int P>
 __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
                                                    const uint2* __restrict__ codes,
-                                                   const uint32_t* __restrict__ meds,
                                                    const uint32_t* __restrict__ pos, uint32_t m,
                                                    int max_d, const uint32_t* __restrict__ minkey,
                                                    uint32_t* __restrict__ dom) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
-  constexpr int QCAP = kBlock * (P + 1);
+  constexpr int QCAP = kBlock * 2;
   constexpr uint32_t FULL = (1u << D) - 1u;
   __shared__ uint32_t A0[P][W];
   __shared__ uint32_t A1[P][W];
@@ -564,36 +564,41 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
   __syncthreads();
   close_down<D, P, W>(A0);
 
-  // pass 1b: code test, queue, exact test of the survivors
+  // pass 1b: code test, queue, exact test of the survivors.  Each thread
+  // tests kPer candidates per round; survivors go to the LDS queue, or are
+  // tested on the spot if the queue is full.
+  constexpr int kPer = 4;
   uint2 pc[P];
-  uint32_t pm[P];
 #pragma unroll
-  for (int pp = 0; pp < P; ++pp) {
-    pc[pp] = codes[min(base + pp, m - 1u)];
-    pm[pp] = meds[min(base + pp, m - 1u)];
-  }
-  for (uint32_t c0 = kc; c0 < m; c0 += kBlock) {
-    const uint32_t b = c0 + tid;
-    if (b < m) {
-      const uint32_t qm = meds[b];
-      const uint2 qc = codes[b];
+  for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
+  for (uint32_t c0 = kc; c0 < m; c0 += kBlock * kPer) {
+    uint2 qc[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t b = c0 + r * kBlock + tid;
+      qc[r] = b < m ? codes[b] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t b = c0 + r * kBlock + tid;
+      if (b >= m) continue;
 #pragma unroll
       for (int pp = 0; pp < P; ++pp) {
-        // median-mask bound first, the 3-bit code bound only if it fails
-        const uint32_t ub1 = ~(qm & ~pm[pp]) & FULL;
-        if ((A0[pp][ub1 >> 5] >> (ub1 & 31)) & 1u) continue;
-        uint32_t ub = code_le(qc.x, pc[pp].x);
-        if (D > 8) ub |= code_le(qc.y, pc[pp].y) << 8;
-        ub &= FULL;
-        if (!((A0[pp][ub >> 5] >> (ub & 31)) & 1u)) {
-          const int slot = atomicAdd(&s_cnt, 1);
+        const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
+        if ((A0[pp][ub >> 5] >> (ub & 31)) & 1u) continue;
+        const int slot = atomicAdd(&s_cnt, 1);
+        if (slot < QCAP) {
           queue[slot] = (b << 3) | (uint32_t)pp;
+        } else {
+          uint32_t lt, eq;
+          pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
+          mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
         }
       }
     }
     __syncthreads();
-    const int n = s_cnt;
-    if (n >= kBlock || c0 + kBlock >= m) {  // uniform
+    const int n = min(s_cnt, QCAP);
+    if (n >= kBlock || c0 + kBlock * kPer >= m) {  // uniform
       for (int e = tid; e < n; e += kBlock) {
         const uint32_t v = queue[e];
         const int pp = (int)(v & 7u);

@@ -341,8 +341,7 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       TRY(mark(c, SKY_STAGE_MEMBER));
       constexpr int PP = sky::member_points<D>();
       hipLaunchKernelGGL((sky::k_member<D, PP>), dim3((ne + PP - 1) / PP), dim3(kBlock), 0,
-                         c->stream, P<float>(c->erows), P<uint2>(c->codes),
-                         P<uint32_t>(c->masks), P<uint32_t>(c->pos),
+                         c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
                          ne, (int)max_d, minkey, P<uint32_t>(c->dom));
       LAUNCHCHK();
     });
