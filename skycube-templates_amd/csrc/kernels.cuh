@@ -295,10 +295,15 @@ __device__ __forceinline__ uint32_t scope_mask(uint32_t k, int D, int max_d) {
   return m;
 }
 
-// Column minima of the extended skyline, as order keys (atomicMin), for the
-// boundary columns of the membership kernel.  minkey must start at ~0.
+// Column minima of the extended skyline, as order keys, for the boundary
+// columns of the membership kernel: a block-wide minimum per column, then
+// one atomicMin per block and column.  minkey must start at ~0.
 template <int D>
-__global__ void k_colmin(const float* __restrict__ erows, uint32_t m, uint32_t* __restrict__ minkey) {
+__global__ __launch_bounds__(kBlock) void k_colmin(const float* __restrict__ erows, uint32_t m,
+                                                   uint32_t* __restrict__ minkey) {
+  __shared__ uint32_t red[D];
+  if (threadIdx.x < D) red[threadIdx.x] = 0xFFFFFFFFu;
+  __syncthreads();
   uint32_t best[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) best[k] = 0xFFFFFFFFu;
@@ -307,7 +312,13 @@ __global__ void k_colmin(const float* __restrict__ erows, uint32_t m, uint32_t* 
     for (int k = 0; k < D; ++k) best[k] = min(best[k], order_key(erows[(size_t)i * D + k]));
   }
 #pragma unroll
-  for (int k = 0; k < D; ++k) atomicMin(&minkey[k], best[k]);
+  for (int k = 0; k < D; ++k) {
+    uint32_t v = best[k];
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    if ((threadIdx.x & 63) == 0) atomicMin(&red[k], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < D) atomicMin(&minkey[threadIdx.x], red[threadIdx.x]);
 }
 
 // Rank of the bits of `sub` inside `zm` (a parallel bit extract): bit j of
@@ -340,30 +351,33 @@ constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
 }
 
-// Quantile pivots of each column, as order keys: block = one column; a
-// strided sample of up to 2048 values is ranked by counting and the values
-// at ranks b*S/16 (b = 1..15) become the pivots.  Any pivots give correct
+// Quantile pivots of each column, as order keys: kPivotSplit blocks per
+// column; a strided sample of up to 2048 values is ranked by counting and
+// the values at ranks b*S/16 (b = 1..15) become the pivots.  Any pivots give correct
 // codes; good ones make the codes selective.
+constexpr int kPivotSplit = 8;  // blocks per column of k_pivots
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ erows, uint32_t m,
                                                    uint32_t* __restrict__ piv) {
   constexpr int S = 2048;
   __shared__ uint32_t val[S];
-  __shared__ uint32_t srt[S];
-  const int col = blockIdx.x;
+  const int col = blockIdx.x / kPivotSplit;
+  const int part = blockIdx.x % kPivotSplit;
   const uint32_t cnt = min((uint32_t)S, m);
   for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
     val[i] = order_key(erows[(size_t)((uint64_t)i * m / cnt) * D + col]);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+  // each sample value's rank (ties broken by position, so ranks are
+  // distinct); the values of ranks (b+1)*cnt/16 are the pivots
+  for (uint32_t i = part * kBlock + threadIdx.x; i < cnt; i += kBlock * kPivotSplit) {
     const uint32_t v = val[i];
     uint32_t r = 0;
     for (uint32_t j = 0; j < cnt; ++j) r += (val[j] < v) || (val[j] == v && j < i);
-    srt[r] = v;
+#pragma unroll 1
+    for (int b = 0; b < kCodePivots; ++b)
+      if ((uint32_t)((b + 1) * cnt / (kCodePivots + 1)) == r) piv[col * kCodePivots + b] = v;
   }
-  __syncthreads();
-  if (threadIdx.x < kCodePivots)
-    piv[col * kCodePivots + threadIdx.x] = srt[(threadIdx.x + 1) * cnt / (kCodePivots + 1)];
 }
 
 // 4-bit quantile code of every column (the number of pivots below the

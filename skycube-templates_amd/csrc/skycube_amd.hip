@@ -253,7 +253,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
                        P<uint32_t>(c->s1pid));
     LAUNCHCHK();
     TRY(mark(c, SKY_STAGE_SELF));
-    hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D * sky::kPivotSplit), dim3(kBlock), 0, c->stream,
                        P<float>(c->s1rows), m, P<uint32_t>(c->piv));
     LAUNCHCHK();
     hipLaunchKernelGGL(sky::k_median_masks<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
@@ -331,7 +331,7 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       hipLaunchKernelGGL(sky::k_colmin<D>, dim3(std::min<unsigned>(grid_for(ne), 1024u)),
                          dim3(kBlock), 0, c->stream, P<float>(c->erows), ne, minkey);
       LAUNCHCHK();
-      hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D), dim3(kBlock), 0, c->stream,
+      hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D * sky::kPivotSplit), dim3(kBlock), 0, c->stream,
                          P<float>(c->erows), ne, P<uint32_t>(c->piv));
       LAUNCHCHK();
       hipLaunchKernelGGL(sky::k_codes<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
