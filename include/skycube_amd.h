@@ -134,6 +134,35 @@ int sky_mdmc_hashcube(sky_ctx *ctx, uint64_t *n_groups, uint64_t *n_entries,
                       uint64_t cap_entries);
 
 /*
+ * Batched MDMC: n_inst independent skycubes in one pass.  Instance t has
+ * n[t] rows of d[t] columns (1..16) and subspace cap max_d[t] (0 = d[t]).
+ * rows holds the instances' row-major matrices concatenated in instance
+ * order, on the host (rows_on_device = 0) or in device memory (1).
+ *
+ * Instances of equal d form a group; every stage runs on all instances of a
+ * group in one launch, and the membership stage is one persistent launch
+ * per group fed by a device work queue, so the launch count does not grow
+ * with n_inst and no host synchronisation happens between stages.  With
+ * use_graph != 0 the whole pass is captured in a HIP graph on first use and
+ * replayed while the batch (pointer, counts, dimensions, caps) is the same.
+ * *total_ext receives the number of extended-skyline points of all
+ * instances.  No reference counterpart: the reference computes one skycube
+ * per process (src/main.h:152-356).
+ */
+int sky_batch_mdmc(sky_ctx *ctx, const float *rows, int rows_on_device,
+                   uint32_t n_inst, const uint32_t *n, const uint32_t *d,
+                   const uint32_t *max_d, int use_graph, uint64_t *total_ext);
+
+/* Results of the last batch, in instance order: ext_off and dom_off
+ * (n_inst + 1 entries each) delimit instance t's extended-skyline row ids
+ * (local, ascending) in ext_pids and its bitmaps (sky_words_per_point(d[t])
+ * words per point) in dom_words.  With NULL ext_pids / dom_words only the
+ * offsets are written. */
+int sky_batch_fetch(sky_ctx *ctx, uint64_t *ext_off, uint32_t *ext_pids,
+                    uint64_t *dom_off, uint32_t *dom_words, uint64_t cap_ext,
+                    uint64_t cap_words);
+
+/*
  * One-cuboid SkyAlign: sky(s) and ext(s) \ sky(s) over a candidate set.
  * Replaces run_skyalign(full_dims, n, dimension_set, sky, ext, working_set,
  * conf) (src/sdsc/skyAlign.cu:9-53, src/sdsc/skyAlign.cuh:321-322).

@@ -1,0 +1,325 @@
+// batch.cuh -- many independent skycube instances in one pass of launches.
+//
+// Instances of one dimensionality D form a group; a group's points are
+// contiguous, instance t owning points off[t] .. off[t+1].  Every stage is
+// the single-instance stage applied per segment, with the segment found from
+// the offsets; counts that only the device knows (prefilter survivors,
+// extended-skyline sizes) stay on the device, so a whole batch enqueues with
+// no host synchronisation and can be captured in a HIP graph.  The
+// membership stage is one persistent launch per group fed by a device work
+// queue of (instance, block of points) tasks.
+#pragma once
+
+#include "kernels.cuh"
+
+namespace sky {
+
+// Segment of index i: the largest t < T with off[t] <= i (off ascending,
+// T + 1 entries); empty segments are skipped over.
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* __restrict__ off, uint32_t T,
+                                           uint32_t i) {
+  uint32_t lo = 0, hi = T;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Graph-safe fills (a captured graph gets no memset nodes).
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t* __restrict__ p, uint64_t n, uint8_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// The group's rows, contiguous: point i of the group is row i - off[t] of
+// instance t, found at float offset src_foff[t] of the caller's array.
+template <int D>
+__global__ void k_bcopy(const float* __restrict__ src, const uint64_t* __restrict__ src_foff,
+                        const uint32_t* __restrict__ off, uint32_t T, uint32_t N,
+                        float* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t t = seg_of(off, T, i);
+  const float* row = src + src_foff[t] + (size_t)(i - off[t]) * D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) dst[(size_t)i * D + k] = row[k];
+}
+
+// Sort keys: instance in the high word, the Manhattan-norm key in the low
+// word (k_score's key), so one radix sort orders every instance by norm.
+template <int D>
+__global__ void k_bscore(const float* __restrict__ rows, const uint32_t* __restrict__ off,
+                         uint32_t T, uint32_t N, uint64_t* __restrict__ key,
+                         uint32_t* __restrict__ val, uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t t = seg_of(off, T, i);
+  float sum = 0.0f;
+  bool nan = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const float v = rows[(size_t)i * D + k];
+    nan |= __builtin_isnan(v);
+    sum += v;
+  }
+  if (nan) atomicOr(err, 2u);
+  key[i] = ((uint64_t)t << 32) | order_key(sum);
+  val[i] = i;
+}
+
+template <int D>
+__global__ void k_bgather(const float* __restrict__ rows, const uint64_t* __restrict__ skey,
+                          const uint32_t* __restrict__ sval, uint32_t N, float* __restrict__ srows,
+                          uint32_t* __restrict__ skeys, uint32_t* __restrict__ spid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t g = sval[i];
+#pragma unroll
+  for (int k = 0; k < D; ++k) srows[(size_t)i * D + k] = rows[(size_t)g * D + k];
+  skeys[i] = (uint32_t)skey[i];
+  spid[i] = g;
+}
+
+// Prefilter, one workgroup per task (instance t, first position): the
+// points of the task against the instance's kPruners lowest-norm points.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bprune(const float* __restrict__ srows,
+                                                   const uint32_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ off,
+                                                   const uint2* __restrict__ tasks,
+                                                   uint8_t* __restrict__ alive) {
+  __shared__ float tile[kBlock * D];
+  __shared__ uint32_t tkey[kBlock];
+  const uint2 task = tasks[blockIdx.x];
+  const uint32_t b0 = off[task.x], nt = off[task.x + 1] - b0;
+  const uint32_t i = task.y + threadIdx.x;
+  const bool valid = i < b0 + nt;
+  float p[D];
+  uint32_t kp = 0;
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) p[k] = srows[(size_t)i * D + k];
+    kp = skeys[i];
+  }
+  const uint32_t npr = min(nt, kPruners);
+  bool dead = false;
+  for (uint32_t t0 = 0; t0 < npr; t0 += kBlock) {
+    const uint32_t cnt = min((uint32_t)kBlock, npr - t0);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < cnt * D; e += kBlock)
+      tile[e] = srows[(size_t)(b0 + t0) * D + e];
+    if (threadIdx.x < cnt) tkey[threadIdx.x] = skeys[b0 + t0 + threadIdx.x];
+    __syncthreads();
+    if (valid && !dead) {
+      for (uint32_t t = 0; t < cnt; ++t) {
+        if (tkey[t] > kp) break;
+        if (b0 + t0 + t == i) continue;
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < D; ++k) all &= tile[t * D + k] < p[k];
+        if (all) {
+          dead = true;
+          break;
+        }
+      }
+    }
+  }
+  if (valid) alive[i] = dead ? 0 : 1;
+}
+
+// Gathers at the positions idx[j], j < *count.
+template <int D>
+__global__ void k_bgather_idx(const float* __restrict__ srows, const uint32_t* __restrict__ skeys,
+                              const uint32_t* __restrict__ spid, const uint32_t* __restrict__ idx,
+                              const uint32_t* __restrict__ count, float* __restrict__ orows,
+                              uint32_t* __restrict__ okeys, uint32_t* __restrict__ opid) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *count) return;
+  const uint32_t i = idx[j];
+#pragma unroll
+  for (int k = 0; k < D; ++k) orows[(size_t)j * D + k] = srows[(size_t)i * D + k];
+  if (okeys) okeys[j] = skeys[i];
+  opid[j] = spid[i];
+}
+
+// Quantile pivots per instance and column (block = (t, column)); a strided
+// sample of up to 256 of the instance's rows is ranked by counting.
+// Instances without rows get pivots above every value.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bpivots(const float* __restrict__ rows,
+                                                    const uint32_t* __restrict__ off, uint32_t T,
+                                                    uint32_t* __restrict__ piv) {
+  constexpr int S = 256;
+  __shared__ uint32_t val[S];
+  const uint32_t t = blockIdx.x / D;
+  const int col = blockIdx.x % D;
+  const uint32_t b0 = off[t], m = off[t + 1] - b0;
+  uint32_t* out = piv + ((size_t)t * 16 + col) * kCodePivots;
+  if (m == 0) {
+    if (threadIdx.x < kCodePivots) out[threadIdx.x] = 0xFFFFFFFFu;
+    return;
+  }
+  const uint32_t cnt = min((uint32_t)S, m);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
+    val[i] = order_key(rows[(size_t)(b0 + (uint64_t)i * m / cnt) * D + col]);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    const uint32_t v = val[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < cnt; ++j) r += (val[j] < v) || (val[j] == v && j < i);
+#pragma unroll 1
+    for (int b = 0; b < kCodePivots; ++b)
+      if ((uint32_t)((b + 1) * cnt / (kCodePivots + 1)) == r) out[b] = v;
+  }
+}
+
+template <int D>
+__global__ void k_bmasks(const float* __restrict__ rows, const uint32_t* __restrict__ off,
+                         uint32_t T, const uint32_t* __restrict__ count,
+                         const uint32_t* __restrict__ piv, uint32_t* __restrict__ masks) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *count) return;
+  const uint32_t t = seg_of(off, T, j);
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    w |= (uint32_t)(order_key(rows[(size_t)j * D + k]) >
+                    piv[((size_t)t * 16 + k) * kCodePivots + 7]) << k;
+  masks[j] = w;
+}
+
+template <int D>
+__global__ void k_bcodes(const float* __restrict__ erows, const uint32_t* __restrict__ off,
+                         uint32_t T, const uint32_t* __restrict__ count,
+                         const uint32_t* __restrict__ piv, uint2* __restrict__ codes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  const uint32_t t = seg_of(off, T, i);
+  uint32_t pl[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const uint32_t key = order_key(erows[(size_t)i * D + k]);
+    const uint32_t* pv = piv + ((size_t)t * 16 + k) * kCodePivots;
+    uint32_t c = 0;
+#pragma unroll
+    for (int b = 0; b < kCodePivots; ++b) c += pv[b] < key;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pl[b] |= ((c >> b) & 1u) << k;
+  }
+  codes[i] = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+}
+
+// Survivor pass, one wave64 per survivor j < *count, against the survivors
+// of its own instance (self_scan, shared with the single-instance kernel).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bself(const float* __restrict__ rows,
+                                                  const uint32_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ masks,
+                                                  const uint32_t* __restrict__ off, uint32_t T,
+                                                  const uint32_t* __restrict__ count,
+                                                  uint8_t* __restrict__ status) {
+  __shared__ uint32_t qbuf[kBlock / 64][192];
+  const uint32_t j = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (j >= *count) return;  // whole wave; no block barrier in this kernel
+  const uint32_t t = seg_of(off, T, j);
+  const uint8_t st = self_scan<D>(rows, keys, masks, off[t], off[t + 1], j, 0,
+                                  qbuf[threadIdx.x >> 6]);
+  if ((threadIdx.x & 63) == 0) status[j] = st;
+}
+
+// flag_pt[spid[j]] = 1 and flag_s1[j] = 1 for survivors nothing strictly
+// dominates; flag_s1 is cleared up to N.
+__global__ void k_bext_flags(const uint8_t* __restrict__ status, const uint32_t* __restrict__ spid,
+                             const uint32_t* __restrict__ count, uint32_t N,
+                             uint8_t* __restrict__ flag_pt, uint8_t* __restrict__ flag_s1) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const bool ext = j < *count && status[j] != 2;
+  flag_s1[j] = ext;
+  if (ext) flag_pt[spid[j]] = 1;
+}
+
+__global__ void k_bscatter_rank(const uint32_t* __restrict__ ids,
+                                const uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < *count) rank[ids[i]] = i;
+}
+
+__global__ void k_bpos(const uint32_t* __restrict__ epid, const uint32_t* __restrict__ count,
+                       const uint32_t* __restrict__ rank, uint32_t* __restrict__ pos) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < *count) pos[j] = rank[epid[j]];
+}
+
+// Column minima per instance (block = instance).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bcolmin(const float* __restrict__ erows,
+                                                    const uint32_t* __restrict__ off,
+                                                    uint32_t* __restrict__ minkey) {
+  __shared__ uint32_t red[D];
+  const uint32_t t = blockIdx.x;
+  if (threadIdx.x < D) red[threadIdx.x] = 0xFFFFFFFFu;
+  __syncthreads();
+  uint32_t best[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) best[k] = 0xFFFFFFFFu;
+  for (uint32_t i = off[t] + threadIdx.x; i < off[t + 1]; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) best[k] = min(best[k], order_key(erows[(size_t)i * D + k]));
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    uint32_t v = best[k];
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    if ((threadIdx.x & 63) == 0) atomicMin(&red[k], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) minkey[(size_t)t * 16 + threadIdx.x] = threadIdx.x < D ? red[threadIdx.x] : 0u;
+}
+
+// Per-instance task counts (blocks of P points) with a zero past the end,
+// for the exclusive scan that numbers the membership tasks.
+__global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t P,
+                         uint32_t* __restrict__ cnt) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > T) return;
+  cnt[t] = t < T ? (off[t + 1] - off[t] + P - 1) / P : 0u;
+}
+
+// Membership, persistent: each workgroup takes (instance, block) tasks from
+// a device counter until the group's tasks (toff[T]) are done.
+template <int D, int P>
+__global__ __launch_bounds__(kBlock) void k_bmember(const float* __restrict__ erows,
+                                                    const uint2* __restrict__ codes,
+                                                    const uint32_t* __restrict__ pos,
+                                                    const uint32_t* __restrict__ eoff, uint32_t T,
+                                                    const uint32_t* __restrict__ toff,
+                                                    const uint32_t* __restrict__ maxd,
+                                                    const uint32_t* __restrict__ minkey,
+                                                    uint32_t* __restrict__ dom,
+                                                    uint32_t* __restrict__ work) {
+  __shared__ uint32_t s_task;
+  const uint32_t total = toff[T];
+  while (true) {
+    if (threadIdx.x == 0) s_task = atomicAdd(work, 1u);
+    __syncthreads();
+    const uint32_t task = s_task;
+    __syncthreads();
+    if (task >= total) break;  // uniform
+    const uint32_t t = seg_of(toff, T, task);
+    const uint32_t e0 = eoff[t];
+    member_body<D, P>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
+                      (int)maxd[t], minkey + (size_t)t * 16, dom, (task - toff[t]) * P);
+  }
+}
+
+}  // namespace sky

@@ -156,36 +156,33 @@ __global__ void k_median_masks(const float* __restrict__ rows, uint32_t m,
 // (total_pruning_moved_data_templated_pitch_two_level_full_warps,
 // src/utilities/kernels.cuh:36-411) over the whole candidate set, with its
 // partition containment test (other subset of mine) as the filter.
+// The survivor pass of point i over the candidates lo .. hi-1 (score order,
+// i inside): 2 if one strictly dominates p, else 1 if one dominates p (only
+// when need_dom), else 0.  Called by a whole wave64; q is the wave's queue
+// (192 entries of LDS).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ rows,
-                                                      const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ masks,
-                                                      uint32_t m, int need_dom,
-                                                      uint8_t* __restrict__ status) {
-  constexpr int WAVES = kBlock / 64;
+__device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
+                                             const uint32_t* __restrict__ keys,
+                                             const uint32_t* __restrict__ masks, uint32_t lo0,
+                                             uint32_t hi0, uint32_t i, int need_dom, uint32_t* q) {
   constexpr uint32_t FULL = (1u << D) - 1u;
-  __shared__ uint32_t qbuf[WAVES][192];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const uint32_t i = blockIdx.x * WAVES + wv;
-  if (i >= m) return;  // whole wave; the kernel has no block barrier
   float p[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
   const uint32_t kp = keys[i];
   const uint32_t mp = masks[i];
-  // ub = first position with key > kp (keys ascend)
-  uint32_t lo = i + 1, hi = m;
+  // ub = first position after i with key > kp (keys ascend)
+  uint32_t lo = i + 1, hi = hi0;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (keys[mid] <= kp) lo = mid + 1;
     else hi = mid;
   }
   const uint32_t ub = lo;
-  uint32_t* q = qbuf[wv];
   int qn = 0;
   uint8_t st = 0;
-  for (uint32_t b0 = 0; b0 < ub; b0 += 128) {
+  for (uint32_t b0 = lo0; b0 < ub; b0 += 128) {
     const uint32_t t0 = b0 + lane, t1 = t0 + 64;
     const uint32_t m0 = t0 < ub ? masks[t0] : 0xFFFFFFFFu;
     const uint32_t m1 = t1 < ub ? masks[t1] : 0xFFFFFFFFu;
@@ -222,7 +219,21 @@ __global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ 
       if (need_dom && __ballot(dom)) st = 1;
     }
   }
-  if (lane == 0) status[i] = st;
+  return st;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ rows,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ masks,
+                                                      uint32_t m, int need_dom,
+                                                      uint8_t* __restrict__ status) {
+  __shared__ uint32_t qbuf[kBlock / 64][192];
+  const int wv = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * (kBlock / 64) + wv;
+  if (i >= m) return;  // whole wave; the kernel has no block barrier
+  const uint8_t st = self_scan<D>(rows, keys, masks, 0, m, i, need_dom, qbuf[wv]);
+  if ((threadIdx.x & 63) == 0) status[i] = st;
 }
 
 // flag[pid[j]] = 1 for every survivor j that nothing strictly dominates.
@@ -507,12 +518,15 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // skipped.  Survivors of the code
 // test are compacted into an LDS queue and get the exact FP32 test together.
 // Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
+//
+// member_body processes the points base .. base+P-1 of one instance; the
+// single-instance kernel and the batched persistent kernel both call it.
 template <int D, int P>
-__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
-                                                   const uint2* __restrict__ codes,
-                                                   const uint32_t* __restrict__ pos, uint32_t m,
-                                                   int max_d, const uint32_t* __restrict__ minkey,
-                                                   uint32_t* __restrict__ dom) {
+__device__ __forceinline__ void member_body(const float* __restrict__ erows,
+                                            const uint2* __restrict__ codes,
+                                            const uint32_t* __restrict__ pos, uint32_t m,
+                                            int max_d, const uint32_t* __restrict__ minkey,
+                                            uint32_t* __restrict__ dom, uint32_t base) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
@@ -531,7 +545,6 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
   uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
   constexpr int XCAP = QCAP * 2 < kXCap ? QCAP * 2 : kXCap;
   const int tid = threadIdx.x;
-  const uint32_t base = blockIdx.x * P;
 
   if (tid < P) {
     const uint32_t a = min(base + tid, m - 1u);
@@ -752,6 +765,16 @@ __global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ ero
     const uint32_t row = pos[base + pp];
     for (int k = tid; k < W; k += kBlock) dom[(size_t)row * W + k] = A1[pp][k];
   }
+  __syncthreads();  // the LDS may be reused by the caller's next block
+}
+
+template <int D, int P>
+__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
+                                                   const uint2* __restrict__ codes,
+                                                   const uint32_t* __restrict__ pos, uint32_t m,
+                                                   int max_d, const uint32_t* __restrict__ minkey,
+                                                   uint32_t* __restrict__ dom) {
+  member_body<D, P>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P);
 }
 
 // ---------------------------------------------------------------------------

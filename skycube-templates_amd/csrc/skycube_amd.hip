@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/skycube_amd.h"
+#include "batch.cuh"
 #include "kernels.cuh"
 
 using sky::DimList;
@@ -25,6 +26,31 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+};
+
+// One dimensionality class of a batch: its instances and device state.
+struct BatchGroup {
+  int D = 0;
+  std::vector<uint32_t> inst;  // caller's instance ids, in group order
+  std::vector<uint32_t> off;   // point offsets of the instances (T + 1)
+  uint32_t T = 0, N = 0, ntasks = 0;
+  DevBuf d_off, d_src_foff, d_tasks, d_maxd;  // descriptors
+  DevBuf ext_pids, eoff, dom, cnt;            // results; cnt: counters
+};
+
+struct BatchState {
+  std::vector<uint32_t> sig_n, sig_d, sig_m;
+  const float* sig_rows = nullptr;
+  std::vector<BatchGroup> groups;
+  uint32_t n_inst = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ready = false;
+  // shared scratch, sized for the largest group
+  DevBuf rows, key_a, key_b, val_a, val_b, srows, skeys, spid, alive, s1idx, s1rows, s1keys,
+      s1pid, masks, status, flag_pt, flag_s1, eidx, erows, epid, pos, codes, rank, piv, minkey,
+      seg_a, seg_b, tmp;
+  size_t tmp_bytes = 0;
 };
 
 }  // namespace
@@ -40,6 +66,9 @@ struct sky_ctx {
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
       masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
       hc_gv, hc_goff;
+  // the last batch
+  BatchState batch;
+  bool has_batch = false;
   // grouping of the last HashCube export
   bool has_hc = false;
   uint64_t hc_groups = 0, hc_entries = 0;
@@ -501,6 +530,304 @@ int build_hashcube(sky_ctx* c) {
   return SKY_OK;
 }
 
+void free_buf(sky_ctx* c, DevBuf& b) {
+  if (b.p) {
+    (void)hipFree(b.p);
+    c->in_use -= b.cap;
+  }
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+void free_batch(sky_ctx* c) {
+  BatchState& B = c->batch;
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+  for (BatchGroup& g : B.groups) {
+    DevBuf* gb[] = {&g.d_off, &g.d_src_foff, &g.d_tasks, &g.d_maxd, &g.ext_pids, &g.eoff, &g.dom,
+                    &g.cnt};
+    for (DevBuf* b : gb) free_buf(c, *b);
+  }
+  B.groups.clear();
+  B.ready = false;
+}
+
+constexpr int kGroupCounters = 8;  // err, survivors, ext (id order), ext (score order), work
+
+int fill32(sky_ctx* c, void* p, uint64_t n, uint32_t v) {
+  if (!n) return SKY_OK;
+  hipLaunchKernelGGL(sky::k_fill_u32, dim3((unsigned)std::min<uint64_t>(grid_for(n), 4096)),
+                     dim3(kBlock), 0, c->stream, static_cast<uint32_t*>(p), n, v);
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
+int fill8(sky_ctx* c, void* p, uint64_t n, uint8_t v) {
+  if (!n) return SKY_OK;
+  hipLaunchKernelGGL(sky::k_fill_u8, dim3((unsigned)std::min<uint64_t>(grid_for(n), 4096)),
+                     dim3(kBlock), 0, c->stream, static_cast<uint8_t*>(p), n, v);
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
+// Temp bytes of every rocPRIM call of a group of N points, T instances.
+int batch_tmp_bytes(sky_ctx* c, uint32_t N, uint32_t T, size_t* out) {
+  size_t mx = 0, b = 0;
+  hipcub::CountingInputIterator<uint32_t> it(0);
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                            (uint32_t*)nullptr, (uint32_t*)nullptr, N, 0, 64,
+                                            c->stream));
+  mx = std::max(mx, b);
+  HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b, it, (uint8_t*)nullptr, (uint32_t*)nullptr,
+                                        (uint32_t*)nullptr, N, c->stream));
+  mx = std::max(mx, b);
+  hipcub::TransformInputIterator<uint32_t, hipcub::CastOp<uint32_t>, const uint8_t*> cast(
+      nullptr, hipcub::CastOp<uint32_t>());
+  HIPCHK(hipcub::DeviceSegmentedReduce::Sum(nullptr, b, cast, (uint32_t*)nullptr, T,
+                                            (const uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                            c->stream));
+  mx = std::max(mx, b);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                          T + 1, c->stream));
+  mx = std::max(mx, b);
+  *out = mx;
+  return SKY_OK;
+}
+
+// Builds the groups of a batch and every device buffer it needs.
+int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32_t* n,
+                const uint32_t* d, const uint32_t* max_d) {
+  free_batch(c);
+  BatchState& B = c->batch;
+  B.n_inst = n_inst;
+  std::vector<uint64_t> foff(n_inst + 1, 0);
+  for (uint32_t t = 0; t < n_inst; ++t) foff[t + 1] = foff[t] + (uint64_t)n[t] * d[t];
+  uint32_t maxN = 0, maxT = 0;
+  for (int D = 1; D <= 16; ++D) {
+    BatchGroup g;
+    g.D = D;
+    for (uint32_t t = 0; t < n_inst; ++t)
+      if ((int)d[t] == D && n[t] > 0) g.inst.push_back(t);
+    if (g.inst.empty()) continue;
+    g.T = (uint32_t)g.inst.size();
+    g.off.assign(g.T + 1, 0);
+    std::vector<uint64_t> sf(g.T);
+    std::vector<uint32_t> md(g.T);
+    std::vector<uint2> tasks;
+    for (uint32_t k = 0; k < g.T; ++k) {
+      const uint32_t t = g.inst[k];
+      if ((uint64_t)g.off[k] + n[t] > 0xFFFFFFFFull)
+        return fail(c, SKY_ERR_ARG, "batch group of d=%d exceeds 2^32 points", D);
+      g.off[k + 1] = g.off[k] + n[t];
+      sf[k] = foff[t];
+      md[k] = (max_d && max_d[t] && max_d[t] < (uint32_t)D) ? max_d[t] : (uint32_t)D;
+      for (uint32_t s0 = 0; s0 < n[t]; s0 += kBlock) tasks.push_back(make_uint2(k, g.off[k] + s0));
+    }
+    g.N = g.off[g.T];
+    g.ntasks = (uint32_t)tasks.size();
+    TRY(ensure(c, g.d_off, sizeof(uint32_t) * (g.T + 1)));
+    TRY(ensure(c, g.d_src_foff, sizeof(uint64_t) * g.T));
+    TRY(ensure(c, g.d_tasks, sizeof(uint2) * tasks.size()));
+    TRY(ensure(c, g.d_maxd, sizeof(uint32_t) * g.T));
+    TRY(ensure(c, g.ext_pids, sizeof(uint32_t) * g.N));
+    TRY(ensure(c, g.eoff, sizeof(uint32_t) * (g.T + 1)));
+    TRY(ensure(c, g.dom, sizeof(uint32_t) * (size_t)g.N * sky_words_per_point(D)));
+    TRY(ensure(c, g.cnt, sizeof(uint32_t) * kGroupCounters));
+    HIPCHK(hipMemcpyAsync(g.d_off.p, g.off.data(), sizeof(uint32_t) * (g.T + 1),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(g.d_src_foff.p, sf.data(), sizeof(uint64_t) * g.T,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(g.d_tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(g.d_maxd.p, md.data(), sizeof(uint32_t) * g.T, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    maxN = std::max(maxN, g.N);
+    maxT = std::max(maxT, g.T);
+    B.groups.push_back(std::move(g));
+  }
+  const size_t N = maxN, T = maxT;
+  TRY(ensure(c, B.rows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.key_a, sizeof(uint64_t) * N));
+  TRY(ensure(c, B.key_b, sizeof(uint64_t) * N));
+  TRY(ensure(c, B.val_a, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.val_b, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.srows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.skeys, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.spid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.alive, N));
+  TRY(ensure(c, B.s1idx, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.s1rows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.s1keys, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.s1pid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.masks, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.status, N));
+  TRY(ensure(c, B.flag_pt, N));
+  TRY(ensure(c, B.flag_s1, N));
+  TRY(ensure(c, B.eidx, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.erows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.epid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.pos, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.codes, sizeof(uint2) * N));
+  TRY(ensure(c, B.rank, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.piv, sizeof(uint32_t) * T * 16 * sky::kCodePivots));
+  TRY(ensure(c, B.minkey, sizeof(uint32_t) * T * 16));
+  TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
+  size_t tb = 0;
+  for (const BatchGroup& g : B.groups) {
+    size_t b = 0;
+    TRY(batch_tmp_bytes(c, g.N, g.T, &b));
+    tb = std::max(tb, b);
+  }
+  TRY(ensure(c, B.tmp, tb));
+  B.tmp_bytes = tb;
+  B.sig_rows = rows_dev;
+  B.sig_n.assign(n, n + n_inst);
+  B.sig_d.assign(d, d + n_inst);
+  B.sig_m.assign(n_inst, 0);
+  if (max_d)
+    for (uint32_t t = 0; t < n_inst; ++t) B.sig_m[t] = max_d[t];
+  B.ready = true;
+  return SKY_OK;
+}
+
+// Segmented sum of 0/1 flags per instance into cnt[0..T) with cnt[T] = 0,
+// then an exclusive scan into off[0..T].
+int seg_count_scan(sky_ctx* c, const uint8_t* flags, const uint32_t* seg, uint32_t T,
+                   uint32_t* cnt, uint32_t* off) {
+  BatchState& B = c->batch;
+  TRY(fill32(c, cnt, T + 1, 0));
+  hipcub::TransformInputIterator<uint32_t, hipcub::CastOp<uint32_t>, const uint8_t*> cast(
+      flags, hipcub::CastOp<uint32_t>());
+  size_t b = B.tmp_bytes;
+  HIPCHK(hipcub::DeviceSegmentedReduce::Sum(B.tmp.p, b, cast, cnt, T, seg, seg + 1, c->stream));
+  b = B.tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, b, cnt, off, T + 1, c->stream));
+  return SKY_OK;
+}
+
+int select_dev(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t* count) {
+  hipcub::CountingInputIterator<uint32_t> it(0);
+  size_t b = c->batch.tmp_bytes;
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->batch.tmp.p, b, it, flags, out, count, n, c->stream));
+  return SKY_OK;
+}
+
+// Enqueues one group's whole pass on the context stream (no host sync, no
+// allocation, no memset: graph-capturable).
+template <int D>
+int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
+  BatchState& B = c->batch;
+  const uint32_t N = g.N, T = g.T;
+  uint32_t* cnt = P<uint32_t>(g.cnt);  // [0] err [1] survivors [2] ext ids [3] ext score [4] work
+  const uint32_t* off = P<uint32_t>(g.d_off);
+  const unsigned gN = grid_for(N);
+  TRY(fill32(c, cnt, kGroupCounters, 0));
+  TRY(fill8(c, B.flag_pt.p, N, 0));
+  hipLaunchKernelGGL(sky::k_bcopy<D>, dim3(gN), dim3(kBlock), 0, c->stream, rows_dev,
+                     P<uint64_t>(g.d_src_foff), off, T, N, P<float>(B.rows));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bscore<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.rows), off,
+                     T, N, P<uint64_t>(B.key_a), P<uint32_t>(B.val_a), cnt);
+  LAUNCHCHK();
+  int tbits = 0;
+  while ((1u << tbits) < T) ++tbits;
+  size_t b = B.tmp_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, b, P<uint64_t>(B.key_a), P<uint64_t>(B.key_b),
+                                            P<uint32_t>(B.val_a), P<uint32_t>(B.val_b), N, 0,
+                                            32 + tbits, c->stream));
+  hipLaunchKernelGGL(sky::k_bgather<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.rows),
+                     P<uint64_t>(B.key_b), P<uint32_t>(B.val_b), N, P<float>(B.srows),
+                     P<uint32_t>(B.skeys), P<uint32_t>(B.spid));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bprune<D>, dim3(g.ntasks), dim3(kBlock), 0, c->stream,
+                     P<float>(B.srows), P<uint32_t>(B.skeys), off, P<uint2>(g.d_tasks),
+                     P<uint8_t>(B.alive));
+  LAUNCHCHK();
+  TRY(select_dev(c, P<uint8_t>(B.alive), N, P<uint32_t>(B.s1idx), cnt + 1));
+  uint32_t* s1off = P<uint32_t>(B.seg_b);
+  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off));
+  hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<float>(B.srows), P<uint32_t>(B.skeys), P<uint32_t>(B.spid),
+                     P<uint32_t>(B.s1idx), cnt + 1, P<float>(B.s1rows), P<uint32_t>(B.s1keys),
+                     P<uint32_t>(B.s1pid));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
+                     P<float>(B.s1rows), s1off, T, P<uint32_t>(B.piv));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.s1rows),
+                     s1off, T, cnt + 1, P<uint32_t>(B.piv), P<uint32_t>(B.masks));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bself<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
+                     P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint32_t>(B.masks), s1off, T,
+                     cnt + 1, P<uint8_t>(B.status));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bext_flags, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint8_t>(B.status), P<uint32_t>(B.s1pid), cnt + 1, N,
+                     P<uint8_t>(B.flag_pt), P<uint8_t>(B.flag_s1));
+  LAUNCHCHK();
+  // extended skyline in id order (the result) and in score order
+  TRY(select_dev(c, P<uint8_t>(B.flag_pt), N, P<uint32_t>(g.ext_pids), cnt + 2));
+  uint32_t* eoff = P<uint32_t>(g.eoff);
+  TRY(seg_count_scan(c, P<uint8_t>(B.flag_pt), off, T, P<uint32_t>(B.seg_a), eoff));
+  TRY(select_dev(c, P<uint8_t>(B.flag_s1), N, P<uint32_t>(B.eidx), cnt + 3));
+  hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<float>(B.s1rows), nullptr, P<uint32_t>(B.s1pid), P<uint32_t>(B.eidx),
+                     cnt + 3, P<float>(B.erows), nullptr, P<uint32_t>(B.epid));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bscatter_rank, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(g.ext_pids), cnt + 2, P<uint32_t>(B.rank));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bpos, dim3(gN), dim3(kBlock), 0, c->stream, P<uint32_t>(B.epid),
+                     cnt + 3, P<uint32_t>(B.rank), P<uint32_t>(B.pos));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bcolmin<D>, dim3(T), dim3(kBlock), 0, c->stream, P<float>(B.erows),
+                     eoff, P<uint32_t>(B.minkey));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
+                     P<float>(B.erows), eoff, T, P<uint32_t>(B.piv));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.erows),
+                     eoff, T, cnt + 3, P<uint32_t>(B.piv), P<uint2>(B.codes));
+  LAUNCHCHK();
+  constexpr int PP = sky::member_points<D>();
+  hipLaunchKernelGGL(sky::k_btasks, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream, eoff, T,
+                     (uint32_t)PP, P<uint32_t>(B.seg_a));
+  LAUNCHCHK();
+  {
+    size_t bb = B.tmp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, bb, P<uint32_t>(B.seg_a),
+                                            P<uint32_t>(B.seg_b), T + 1, c->stream));
+  }
+  hipLaunchKernelGGL((sky::k_bmember<D, PP>), dim3(1024), dim3(kBlock), 0, c->stream,
+                     P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
+                     P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
+                     P<uint32_t>(g.dom), cnt + 4);
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
+int batch_enqueue(sky_ctx* c, const float* rows_dev) {
+  for (BatchGroup& g : c->batch.groups) {
+    int rc = SKY_OK;
+    SKY_DISPATCH_D(g.D, { rc = batch_group<D>(c, g, rows_dev); });
+    if (rc != SKY_OK) return rc;
+  }
+  return SKY_OK;
+}
+
+bool batch_same(const BatchState& B, const float* rows, uint32_t n_inst, const uint32_t* n,
+                const uint32_t* d, const uint32_t* max_d) {
+  if (!B.ready || B.sig_rows != rows || B.n_inst != n_inst) return false;
+  for (uint32_t t = 0; t < n_inst; ++t)
+    if (B.sig_n[t] != n[t] || B.sig_d[t] != d[t] || B.sig_m[t] != (max_d ? max_d[t] : 0u))
+      return false;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -539,6 +866,7 @@ void sky_ctx_destroy(sky_ctx* c) {
                     &c->hc_goff};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
+  free_batch(c);
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -638,6 +966,111 @@ int sky_mdmc_hashcube(sky_ctx* c, uint64_t* n_groups, uint64_t* n_entries, uint3
     group_off[0] = 0;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
+  return SKY_OK;
+}
+
+int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n_inst,
+                   const uint32_t* n, const uint32_t* d, const uint32_t* max_d, int use_graph,
+                   uint64_t* total_ext) {
+  TRY(check_ctx(c));
+  if (!n || !d || !total_ext || (n_inst && !rows)) return fail(c, SKY_ERR_ARG, "null argument");
+  uint64_t total_f = 0;
+  for (uint32_t t = 0; t < n_inst; ++t) {
+    if (d[t] < 1 || d[t] > 16) return fail(c, SKY_ERR_ARG, "instance %u: d=%u outside [1,16]", t, d[t]);
+    total_f += (uint64_t)n[t] * d[t];
+  }
+  c->has_mdmc = false;
+  c->has_hc = false;
+  c->has_batch = false;
+  const float* rows_dev = rows;
+  if (!rows_on_device) {
+    TRY(ensure(c, c->rows, sizeof(float) * std::max<uint64_t>(total_f, 1)));
+    if (total_f)
+      HIPCHK(hipMemcpyAsync(c->rows.p, rows, sizeof(float) * total_f, hipMemcpyHostToDevice,
+                            c->stream));
+    rows_dev = P<float>(c->rows);
+  }
+  BatchState& B = c->batch;
+  if (!batch_same(B, rows_dev, n_inst, n, d, max_d)) TRY(batch_setup(c, rows_dev, n_inst, n, d, max_d));
+  if (use_graph) {
+    if (!B.exec) {
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_enqueue(c, rows_dev);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc != SKY_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess)
+        return fail(c, SKY_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+      B.graph = graph;
+      HIPCHK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(B.exec, c->stream));
+  } else {
+    TRY(batch_enqueue(c, rows_dev));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  uint64_t tot = 0;
+  for (BatchGroup& g : B.groups) {
+    uint32_t h[kGroupCounters];
+    HIPCHK(hipMemcpy(h, g.cnt.p, sizeof(h), hipMemcpyDeviceToHost));
+    if (h[0] & 2u) return fail(c, SKY_ERR_NAN, "input holds NaN");
+    tot += h[2];
+  }
+  *total_ext = tot;
+  c->has_batch = true;
+  return SKY_OK;
+}
+
+int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t* dom_off,
+                    uint32_t* dom_words, uint64_t cap_ext, uint64_t cap_words) {
+  TRY(check_ctx(c));
+  if (!ext_off || !dom_off) return fail(c, SKY_ERR_ARG, "null argument");
+  if (!c->has_batch) return fail(c, SKY_ERR_STATE, "no batch result is held");
+  BatchState& B = c->batch;
+  const uint32_t NI = B.n_inst;
+  std::vector<uint32_t> ne(NI, 0), dim(NI, 0);
+  std::vector<std::vector<uint32_t>> eo(B.groups.size());
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    BatchGroup& g = B.groups[gi];
+    eo[gi].resize(g.T + 1);
+    HIPCHK(hipMemcpy(eo[gi].data(), g.eoff.p, sizeof(uint32_t) * (g.T + 1),
+                     hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < g.T; ++k) {
+      ne[g.inst[k]] = eo[gi][k + 1] - eo[gi][k];
+      dim[g.inst[k]] = (uint32_t)g.D;
+    }
+  }
+  for (uint32_t t = 0; t < NI; ++t) dim[t] = dim[t] ? dim[t] : B.sig_d[t];
+  ext_off[0] = dom_off[0] = 0;
+  for (uint32_t t = 0; t < NI; ++t) {
+    ext_off[t + 1] = ext_off[t] + ne[t];
+    dom_off[t + 1] = dom_off[t] + (uint64_t)ne[t] * sky_words_per_point(dim[t]);
+  }
+  if (!ext_pids || !dom_words) return SKY_OK;
+  if (cap_ext < ext_off[NI] || cap_words < dom_off[NI])
+    return fail(c, SKY_ERR_CAPACITY, "batch needs %llu ids and %llu words",
+                (unsigned long long)ext_off[NI], (unsigned long long)dom_off[NI]);
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    BatchGroup& g = B.groups[gi];
+    const uint32_t tot = eo[gi][g.T];
+    const uint32_t W = sky_words_per_point(g.D);
+    std::vector<uint32_t> ids(tot), words((size_t)tot * W);
+    if (tot) {
+      HIPCHK(hipMemcpy(ids.data(), g.ext_pids.p, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(words.data(), g.dom.p, sizeof(uint32_t) * words.size(),
+                       hipMemcpyDeviceToHost));
+    }
+    for (uint32_t k = 0; k < g.T; ++k) {
+      const uint32_t t = g.inst[k];
+      for (uint32_t j = eo[gi][k]; j < eo[gi][k + 1]; ++j)
+        ext_pids[ext_off[t] + (j - eo[gi][k])] = ids[j] - g.off[k];
+      std::copy(words.begin() + (size_t)eo[gi][k] * W, words.begin() + (size_t)eo[gi][k + 1] * W,
+                dom_words + dom_off[t]);
+    }
+  }
   return SKY_OK;
 }
 

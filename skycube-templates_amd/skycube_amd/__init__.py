@@ -35,7 +35,7 @@ EXPORTS = (
     "sky_ctx_create", "sky_ctx_destroy", "sky_last_error", "sky_words_per_point",
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
-    "sky_ctx_stage_times", "sky_mdmc_hashcube",
+    "sky_ctx_stage_times", "sky_mdmc_hashcube", "sky_batch_mdmc", "sky_batch_fetch",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -89,6 +89,10 @@ def load():
                            ctypes.c_uint64]
     L.sky_mdmc_hashcube.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u32p, _u32p, _u64p, _u32p,
                                      ctypes.c_uint64, ctypes.c_uint64]
+    L.sky_batch_mdmc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                  _u32p, _u32p, _u32p, ctypes.c_int, _u64p]
+    L.sky_batch_fetch.argtypes = [ctypes.c_void_p, _u64p, _u32p, _u64p, _u32p, ctypes.c_uint64,
+                                  ctypes.c_uint64]
     L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     _lib = L
@@ -264,6 +268,50 @@ class Context:
                                               _ptr(gw, _u32p), _ptr(gv, _u32p), _ptr(go, _u64p),
                                               _ptr(ids, _u32p), max(g, 1), max(t, 1)))
         return gw[:g], gv[:g], go, ids[:t]
+
+    def batch_mdmc_raw(self, rows_ptr: int, on_device: bool, n, d, max_d=None,
+                       use_graph: bool = False) -> int:
+        """Runs a batch from a pointer to the concatenated rows; returns the
+        total number of extended-skyline points (results stay on device)."""
+        n = np.ascontiguousarray(n, dtype=np.uint32)
+        d = np.ascontiguousarray(d, dtype=np.uint32)
+        m = None if max_d is None else np.ascontiguousarray(max_d, dtype=np.uint32)
+        tot = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.sky_batch_mdmc(self._h, ctypes.c_void_p(rows_ptr), int(on_device),
+                                           n.size, _ptr(n, _u32p), _ptr(d, _u32p),
+                                           None if m is None else _ptr(m, _u32p),
+                                           int(bool(use_graph)), _ptr(tot, _u64p)))
+        return int(tot[0])
+
+    def batch_fetch(self, n_inst: int):
+        """[(ext_pids, bitmaps)] of the last batch, in instance order."""
+        eo = np.zeros(n_inst + 1, dtype=np.uint64)
+        do = np.zeros(n_inst + 1, dtype=np.uint64)
+        self._check(self._L.sky_batch_fetch(self._h, _ptr(eo, _u64p), None, _ptr(do, _u64p), None,
+                                            0, 0))
+        ids = np.zeros(max(int(eo[-1]), 1), dtype=np.uint32)
+        words = np.zeros(max(int(do[-1]), 1), dtype=np.uint32)
+        self._check(self._L.sky_batch_fetch(self._h, _ptr(eo, _u64p), _ptr(ids, _u32p),
+                                            _ptr(do, _u64p), _ptr(words, _u32p), ids.size,
+                                            words.size))
+        out = []
+        for t in range(n_inst):
+            e = ids[int(eo[t]):int(eo[t + 1])].copy()
+            w = words[int(do[t]):int(do[t + 1])]
+            W = w.size // max(e.size, 1) if e.size else 0
+            out.append((e, w.reshape(e.size, W) if e.size else w.reshape(0, 0)))
+        return out
+
+    def batch_mdmc(self, instances, max_d=None, use_graph: bool = False):
+        """Skycubes of a list of n_t x d_t arrays in one batch:
+        [(ext_pids, bitmaps)] per instance."""
+        mats = [self._rows(x) for x in instances]
+        n = [m.shape[0] for m in mats]
+        d = [m.shape[1] for m in mats]
+        flat = np.ascontiguousarray(np.concatenate([m.ravel() for m in mats]) if mats
+                                    else np.zeros(1, np.float32))
+        self.batch_mdmc_raw(flat.ctypes.data, False, n, d, max_d, use_graph)
+        return self.batch_fetch(len(mats))
 
     def skyalign(self, rows, subspace: int, cand=None):
         """(sky ids ascending, ext-but-not-sky ids ascending) of cuboid `subspace`."""
