@@ -140,6 +140,106 @@ def bench_batch(args, torch, skycube_amd, datagen) -> int:
     return 0
 
 
+def _pct(a, q):
+    return round(float(np.percentile(np.asarray(a), q)), 3) if len(a) else None
+
+
+def bench_serve(args, torch, skycube_amd, datagen) -> int:
+    """Serving workload: a Poisson stream of skycube requests, n log-normal
+    (median 20,000, sigma 1.0, clipped to [500, 500,000]), d in 6..14, mixed
+    distributions (seed 5), submitted from the host at their arrival times
+    to the continuous-batching server.  The rate is 0.7 x the capacity that
+    a closed-loop pass (every request submitted at once) measures first,
+    unless --qps is given.  Latency = arrival -> result in host memory."""
+    import threading
+
+    count = args.requests
+    sizes = datagen.serving_stream(count, 1.0, seed=5)
+    mats = [datagen.by_name(dist, n, d, sd) for (_, n, d, dist, sd) in sizes]
+    pts = int(sum(m.shape[0] for m in mats))
+    srv = skycube_amd.Server(0, hbm_budget=0, workers=args.workers)
+    # warm-up (kernel loading, buffer growth)
+    for i in [srv.submit(m) for m in mats[:16]]:
+        srv.result(i)
+    t0 = time.perf_counter()
+    ids = [srv.submit(m) for m in mats]
+    for i in ids:
+        srv.result(i)
+    t_closed = time.perf_counter() - t0
+    capacity = count / t_closed
+    qps = args.qps or 0.7 * capacity
+    stream = datagen.serving_stream(count, qps, seed=5)
+    st0 = srv.stats()
+    lat, lag, bsz = [None] * count, [0.0] * count, [0] * count
+    ids = [None] * count
+    kept = {}
+    done = threading.Event()
+    errors = []
+
+    def collector():
+        try:
+            for k in range(count):
+                while ids[k] is None:
+                    time.sleep(0.0002)
+                e, dm, ms, b = srv.result(ids[k])
+                lat[k] = ms + lag[k]
+                bsz[k] = b
+                if args.verify_every and k % args.verify_every == 0:
+                    kept[k] = (e, dm)
+        except Exception as exc:  # reported in the JSON line
+            errors.append(repr(exc))
+        done.set()
+
+    th = threading.Thread(target=collector, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for k, (arr, *_rest) in enumerate(stream):
+        now = time.perf_counter() - t0
+        if arr > now:
+            time.sleep(arr - now)
+        lag[k] = max(0.0, (time.perf_counter() - t0 - arr) * 1e3)
+        ids[k] = srv.submit(mats[k])
+    done.wait()
+    t_open = time.perf_counter() - t0
+    st1 = srv.stats()
+    srv.close()
+    busy = (st1["busy_ms"] - st0["busy_ms"]) / (t_open * 1e3)
+    # a sampled subset of the stream's results, each recomputed alone on a
+    # plain context, must match
+    verified = None
+    if kept:
+        ctx = skycube_amd.Context(0)
+        ok = 0
+        for k, (e0, d0) in kept.items():
+            e, dm = ctx.mdmc(mats[k])
+            ok += int(np.array_equal(e, e0) and np.array_equal(dm, d0))
+        ctx.close()
+        verified = f"{ok}/{len(kept)} sampled requests identical to the same rows computed alone"
+    good = [v for v in lat if v is not None]
+    out = {
+        "metric": METRIC, "value": _pct(good, 99), "unit": "ms (p99 latency)", "n_gpus": 1,
+        "steps": count, "warmup": 16, "ms_per_step": round(t_open * 1e3 / count, 3),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {count} requests, Poisson arrivals at {qps:.1f}/s, "
+                               f"n log-normal (median 20000, sigma 1.0, [500, 500000]), "
+                               f"d in 6..14, mixed distributions, seed 5",
+                   "points": pts, "workers": args.workers},
+        "latency_ms": {"p50": _pct(good, 50), "p90": _pct(good, 90), "p99": _pct(good, 99),
+                       "max": _pct(good, 100), "mean": round(float(np.mean(good)), 3)},
+        "requests_per_s": round(count / t_open, 1),
+        "closed_loop_capacity_per_s": round(capacity, 1),
+        "offered_qps": round(qps, 1),
+        "server_busy_frac": round(busy, 3),
+        "batches": st1["batches"] - st0["batches"],
+        "batch_size": {"mean": round(float(np.mean(bsz)), 2), "max": int(max(bsz))},
+        "max_batches_in_flight": st1["max_concurrent"],
+        "verified": verified, "errors": errors or None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if not errors else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,9 +248,16 @@ def main() -> int:
     ap.add_argument("--workload", default="integer_ties")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="sha256 of the result vs the golden digest")
-    ap.add_argument("--mode", default="single", choices=["single", "batch"],
-                    help="single: one skycube per step (the headline); batch: the batched workload")
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
+                    help="single: one skycube per step (the headline); batch: the batched "
+                         "workload; serve: the Poisson request stream")
     ap.add_argument("--instances", type=int, default=4096, help="batch mode: instance count")
+    ap.add_argument("--requests", type=int, default=600, help="serve mode: request count")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve mode: arrival rate "
+                    "(default 0.7 x the closed-loop capacity)")
+    ap.add_argument("--workers", type=int, default=2, help="serve mode: server workers")
+    ap.add_argument("--verify-every", type=int, default=10,
+                    help="serve mode: check every k-th request against a plain call (0: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (for scaling HBM counters)")
     args = ap.parse_args()
@@ -177,6 +284,8 @@ def main() -> int:
 
     if args.mode == "batch":
         return bench_batch(args, torch, skycube_amd, datagen)
+    if args.mode == "serve":
+        return bench_serve(args, torch, skycube_amd, datagen)
 
     w = datagen.WORKLOADS[args.workload]
     n, d, max_d = w["n"], w["d"], w["max_d"]

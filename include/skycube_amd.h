@@ -63,8 +63,14 @@ const char *sky_last_error(const sky_ctx *ctx);
  * bitmap has one extra, always-zero word (src/skycube/NodeAuxilliary.h:31). */
 uint32_t sky_words_per_point(uint32_t d);
 
-/* Device memory currently held by the context, in bytes. */
+/* Device memory currently held by the context, in bytes.  Buffers grow on
+ * demand and are kept for the next call. */
 size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
+
+/* Frees every device buffer the context holds (held results included) and
+ * keeps the context usable.  No reference counterpart (the reference keeps
+ * its GPU state until exit). */
+int sky_ctx_release(sky_ctx *ctx);
 
 /* Device-time breakdown of the last MDMC call, from HIP events recorded on
  * the context's stream while timing is enabled (off by default; it adds a
@@ -195,6 +201,60 @@ int sky_sdsc(sky_ctx *ctx, const float *rows, uint32_t n, uint32_t d,
              uint32_t max_d, uint64_t *sky_off, uint32_t *sky_ids,
              uint64_t sky_cap, uint64_t *ext_off, uint32_t *ext_ids,
              uint64_t ext_cap);
+
+/*
+ * Skycube server: continuous batching of independent skycube requests on one
+ * device.  No reference counterpart -- the reference computes one skycube per
+ * process (src/main.h:152-356); this serves many callers from one process.
+ *
+ * sky_server_submit queues a request (the rows are copied; NaN input and a
+ * request whose device footprint exceeds one worker's share of the budget are
+ * rejected at once with SKY_ERR_NAN / SKY_ERR_OOM_BUDGET).  `workers` threads,
+ * each with its own context and stream, repeatedly take every queued request,
+ * first come first served, up to `max_batch` requests and hbm_budget_bytes /
+ * workers of estimated device memory, and compute them as one batched MDMC
+ * pass (sky_batch_mdmc), so that requests arriving while the device is busy
+ * join the next batch and one worker's copies overlap another's kernels.
+ * Every request's result is identical to sky_mdmc on the same rows alone.
+ * hbm_budget_bytes = 0 means 80% of the device's free memory.
+ */
+typedef struct sky_server sky_server;
+
+int sky_server_create(int device, size_t hbm_budget_bytes, uint32_t workers,
+                      uint32_t max_batch, sky_server **out);
+
+/* Waits for every queued request, then stops the workers and frees all. */
+void sky_server_destroy(sky_server *srv);
+
+/* Estimated device bytes of one request of n points of d dimensions (an
+ * upper bound of what a batched pass allocates for it). */
+uint64_t sky_server_footprint(uint32_t n, uint32_t d);
+
+/* Queues n x d rows (FP32, row-major, host memory) with subspaces of at most
+ * max_d dimensions (0 = d); *id receives the request id. */
+int sky_server_submit(sky_server *srv, const float *rows, uint32_t n,
+                      uint32_t d, uint32_t max_d, uint64_t *id);
+
+/* Blocks until request id is done (timeout_ms < 0: no limit); returns its
+ * status, or SKY_ERR_STATE on time-out or an unknown id. */
+int sky_server_wait(sky_server *srv, uint64_t id, int timeout_ms);
+
+/* Result of a finished request, as sky_mdmc returns it: *n_ext extended-
+ * skyline row ids (ascending) and their bitmaps (n_ext *
+ * sky_words_per_point(d) words, cap_words available).  With NULL ext_pids or
+ * dom_words only the sizes, latency (submit to result ready, ms) and batch
+ * size are written and the result is kept; otherwise it is copied out and
+ * released. */
+int sky_server_result(sky_server *srv, uint64_t id, uint32_t *ext_pids,
+                      uint32_t *n_ext, uint32_t *dom_words, uint64_t cap_words,
+                      double *latency_ms, uint32_t *batch_size);
+
+/* Batches run so far, the most batches that were in flight at once, requests
+ * waiting, and the wall time (ms) during which at least one batch was in
+ * flight (upload, kernels and download). */
+int sky_server_stats(sky_server *srv, uint64_t *batches,
+                     uint64_t *max_concurrent, uint64_t *queued,
+                     double *busy_ms);
 
 #ifdef __cplusplus
 }

@@ -41,7 +41,8 @@ struct BatchGroup {
 struct BatchState {
   std::vector<uint32_t> sig_n, sig_d, sig_m;
   const float* sig_rows = nullptr;
-  std::vector<BatchGroup> groups;
+  BatchGroup pool[17];              // per dimensionality, buffers kept across batches
+  std::vector<BatchGroup*> groups;  // the groups of the current batch, ascending d
   uint32_t n_inst = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -539,19 +540,32 @@ void free_buf(sky_ctx* c, DevBuf& b) {
   b.cap = 0;
 }
 
+// Forgets the current batch (graph included); the buffers stay for the next.
 void free_batch(sky_ctx* c) {
   BatchState& B = c->batch;
   if (B.exec) (void)hipGraphExecDestroy(B.exec);
   if (B.graph) (void)hipGraphDestroy(B.graph);
   B.exec = nullptr;
   B.graph = nullptr;
-  for (BatchGroup& g : B.groups) {
+  B.groups.clear();
+  B.ready = false;
+}
+
+// Frees every device buffer of the batch path.
+void free_batch_buffers(sky_ctx* c) {
+  free_batch(c);
+  BatchState& B = c->batch;
+  for (BatchGroup& g : B.pool) {
     DevBuf* gb[] = {&g.d_off, &g.d_src_foff, &g.d_tasks, &g.d_maxd, &g.ext_pids, &g.eoff, &g.dom,
                     &g.cnt};
     for (DevBuf* b : gb) free_buf(c, *b);
   }
-  B.groups.clear();
-  B.ready = false;
+  DevBuf* sb[] = {&B.rows,  &B.key_a,  &B.key_b,  &B.val_a, &B.val_b,  &B.srows,  &B.skeys,
+                  &B.spid,  &B.alive,  &B.s1idx,  &B.s1rows, &B.s1keys, &B.s1pid, &B.masks,
+                  &B.status, &B.flag_pt, &B.flag_s1, &B.eidx, &B.erows, &B.epid,  &B.pos,
+                  &B.codes, &B.rank,   &B.piv,    &B.minkey, &B.seg_a, &B.seg_b, &B.tmp};
+  for (DevBuf* b : sb) free_buf(c, *b);
+  B.tmp_bytes = 0;
 }
 
 constexpr int kGroupCounters = 8;  // err, survivors, ext (id order), ext (score order), work
@@ -606,8 +620,9 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   for (uint32_t t = 0; t < n_inst; ++t) foff[t + 1] = foff[t] + (uint64_t)n[t] * d[t];
   uint32_t maxN = 0, maxT = 0;
   for (int D = 1; D <= 16; ++D) {
-    BatchGroup g;
+    BatchGroup& g = B.pool[D];
     g.D = D;
+    g.inst.clear();
     for (uint32_t t = 0; t < n_inst; ++t)
       if ((int)d[t] == D && n[t] > 0) g.inst.push_back(t);
     if (g.inst.empty()) continue;
@@ -646,7 +661,7 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
     HIPCHK(hipStreamSynchronize(c->stream));  // host vectors go out of scope
     maxN = std::max(maxN, g.N);
     maxT = std::max(maxT, g.T);
-    B.groups.push_back(std::move(g));
+    B.groups.push_back(&g);
   }
   const size_t N = maxN, T = maxT;
   TRY(ensure(c, B.rows, sizeof(float) * N * 16));
@@ -677,9 +692,9 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
   TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
   size_t tb = 0;
-  for (const BatchGroup& g : B.groups) {
+  for (const BatchGroup* g : B.groups) {
     size_t b = 0;
-    TRY(batch_tmp_bytes(c, g.N, g.T, &b));
+    TRY(batch_tmp_bytes(c, g->N, g->T, &b));
     tb = std::max(tb, b);
   }
   TRY(ensure(c, B.tmp, tb));
@@ -811,9 +826,9 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
 }
 
 int batch_enqueue(sky_ctx* c, const float* rows_dev) {
-  for (BatchGroup& g : c->batch.groups) {
+  for (BatchGroup* g : c->batch.groups) {
     int rc = SKY_OK;
-    SKY_DISPATCH_D(g.D, { rc = batch_group<D>(c, g, rows_dev); });
+    SKY_DISPATCH_D(g->D, { rc = batch_group<D>(c, *g, rows_dev); });
     if (rc != SKY_OK) return rc;
   }
   return SKY_OK;
@@ -842,7 +857,7 @@ int sky_ctx_create(int device, size_t hbm_budget_bytes, sky_ctx** out) {
     return SKY_ERR_DEVICE;
   sky_ctx* c = new sky_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return SKY_ERR_DEVICE;
   }
@@ -853,9 +868,12 @@ int sky_ctx_create(int device, size_t hbm_budget_bytes, sky_ctx** out) {
   return SKY_OK;
 }
 
-void sky_ctx_destroy(sky_ctx* c) {
-  if (!c) return;
-  (void)hipSetDevice(c->device);
+}  // extern "C"
+
+namespace {
+
+// Frees every device buffer the context holds; results are gone afterwards.
+void free_all(sky_ctx* c) {
   DevBuf* bufs[] = {&c->rows,   &c->key_a,  &c->key_b,    &c->pid_a,  &c->pid_b,
                     &c->srows,  &c->alive,  &c->s1idx,    &c->s1rows, &c->s1keys,
                     &c->s1pid,  &c->status, &c->flag,     &c->ext_pids, &c->erows,
@@ -864,13 +882,31 @@ void sky_ctx_destroy(sky_ctx* c) {
                     &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
                     &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
                     &c->hc_goff};
-  for (DevBuf* b : bufs)
-    if (b->p) (void)hipFree(b->p);
-  free_batch(c);
+  for (DevBuf* b : bufs) free_buf(c, *b);
+  free_batch_buffers(c);
+  c->has_mdmc = c->has_hc = c->has_batch = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sky_ctx_destroy(sky_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int sky_ctx_release(sky_ctx* c) {
+  TRY(check_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free_all(c);
+  return SKY_OK;
 }
 
 const char* sky_last_error(const sky_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -1013,11 +1049,14 @@ int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   uint64_t tot = 0;
-  for (BatchGroup& g : B.groups) {
-    uint32_t h[kGroupCounters];
-    HIPCHK(hipMemcpy(h, g.cnt.p, sizeof(h), hipMemcpyDeviceToHost));
-    if (h[0] & 2u) return fail(c, SKY_ERR_NAN, "input holds NaN");
-    tot += h[2];
+  std::vector<uint32_t> h(kGroupCounters * B.groups.size());
+  for (size_t gi = 0; gi < B.groups.size(); ++gi)
+    HIPCHK(hipMemcpyAsync(&h[gi * kGroupCounters], B.groups[gi]->cnt.p,
+                          sizeof(uint32_t) * kGroupCounters, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    if (h[gi * kGroupCounters] & 2u) return fail(c, SKY_ERR_NAN, "input holds NaN");
+    tot += h[gi * kGroupCounters + 2];
   }
   *total_ext = tot;
   c->has_batch = true;
@@ -1034,10 +1073,14 @@ int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t*
   std::vector<uint32_t> ne(NI, 0), dim(NI, 0);
   std::vector<std::vector<uint32_t>> eo(B.groups.size());
   for (size_t gi = 0; gi < B.groups.size(); ++gi) {
-    BatchGroup& g = B.groups[gi];
-    eo[gi].resize(g.T + 1);
-    HIPCHK(hipMemcpy(eo[gi].data(), g.eoff.p, sizeof(uint32_t) * (g.T + 1),
-                     hipMemcpyDeviceToHost));
+    eo[gi].resize(B.groups[gi]->T + 1);
+    HIPCHK(hipMemcpyAsync(eo[gi].data(), B.groups[gi]->eoff.p,
+                          sizeof(uint32_t) * (B.groups[gi]->T + 1), hipMemcpyDeviceToHost,
+                          c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    const BatchGroup& g = *B.groups[gi];
     for (uint32_t k = 0; k < g.T; ++k) {
       ne[g.inst[k]] = eo[gi][k + 1] - eo[gi][k];
       dim[g.inst[k]] = (uint32_t)g.D;
@@ -1054,14 +1097,16 @@ int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t*
     return fail(c, SKY_ERR_CAPACITY, "batch needs %llu ids and %llu words",
                 (unsigned long long)ext_off[NI], (unsigned long long)dom_off[NI]);
   for (size_t gi = 0; gi < B.groups.size(); ++gi) {
-    BatchGroup& g = B.groups[gi];
+    const BatchGroup& g = *B.groups[gi];
     const uint32_t tot = eo[gi][g.T];
     const uint32_t W = sky_words_per_point(g.D);
     std::vector<uint32_t> ids(tot), words((size_t)tot * W);
     if (tot) {
-      HIPCHK(hipMemcpy(ids.data(), g.ext_pids.p, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(words.data(), g.dom.p, sizeof(uint32_t) * words.size(),
-                       hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(ids.data(), g.ext_pids.p, sizeof(uint32_t) * tot,
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(words.data(), g.dom.p, sizeof(uint32_t) * words.size(),
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
     }
     for (uint32_t k = 0; k < g.T; ++k) {
       const uint32_t t = g.inst[k];

@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "SkyError", "Context", "library_path", "load", "words_per_point",
-    "in_scope_mask", "members", "SKY_OK", "hashcube_query", "hashcube_from_bitmaps",
+    "in_scope_mask", "members", "SKY_OK", "hashcube_query", "hashcube_from_bitmaps", "Server",
 ]
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,6 +36,8 @@ EXPORTS = (
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
     "sky_ctx_stage_times", "sky_mdmc_hashcube", "sky_batch_mdmc", "sky_batch_fetch",
+    "sky_ctx_release", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
+    "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -95,6 +97,20 @@ def load():
                                   ctypes.c_uint64]
     L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
+    L.sky_ctx_release.argtypes = [ctypes.c_void_p]
+    L.sky_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+    L.sky_server_destroy.argtypes = [ctypes.c_void_p]
+    L.sky_server_destroy.restype = None
+    L.sky_server_footprint.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.sky_server_footprint.restype = ctypes.c_uint64
+    L.sky_server_submit.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint32, _u64p]
+    L.sky_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
+    L.sky_server_result.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u32p, _u32p, _u32p,
+                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), _u32p]
+    L.sky_server_stats.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p,
+                                   ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -350,3 +366,82 @@ class Context:
                                      _ptr(so, _u64p), _ptr(sky, _u32p), sky.size, _ptr(eo, _u64p),
                                      _ptr(ext, _u32p), ext.size))
         return so, sky[: int(so[S])].copy(), eo, ext[: int(eo[S])].copy()
+
+
+class Server:
+    """Continuous-batching skycube server (sky_server_* of the C ABI).
+
+    ``submit`` queues one skycube request and returns its id at once;
+    ``result`` blocks until it is done and returns (ext_pids, dom, latency_ms,
+    batch_size), the same arrays ``Context.mdmc`` returns for those rows."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2,
+                 max_batch: int = 0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.sky_server_create(device, hbm_budget, workers, max_batch, ctypes.byref(h))
+        if rc != SKY_OK:
+            raise SkyError(rc, "server creation failed")
+        self._h = h
+        self._d = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sky_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def footprint(n: int, d: int) -> int:
+        return int(load().sky_server_footprint(n, d))
+
+    def submit(self, rows, max_d: int | None = None) -> int:
+        x = Context._rows(rows)
+        n, d = x.shape
+        rid = ctypes.c_uint64()
+        rc = self._lib.sky_server_submit(self._h, _ptr(x, _f32p), n, d, max_d or 0,
+                                         ctypes.byref(rid))
+        if rc != SKY_OK:
+            raise SkyError(rc, f"request of {n} x {d} rejected")
+        self._d[rid.value] = d
+        return rid.value
+
+    def result(self, rid: int, timeout_ms: int = -1):
+        rc = self._lib.sky_server_wait(self._h, rid, timeout_ms)
+        if rc != SKY_OK:
+            self._d.pop(rid, None)
+            raise SkyError(rc, f"request {rid} failed or timed out")
+        ne = ctypes.c_uint32()
+        lat = ctypes.c_double()
+        bs = ctypes.c_uint32()
+        rc = self._lib.sky_server_result(self._h, rid, None, ctypes.byref(ne), None, 0,
+                                         ctypes.byref(lat), ctypes.byref(bs))
+        if rc != SKY_OK:
+            raise SkyError(rc, f"request {rid}: no result")
+        W = words_per_point(self._d.pop(rid))
+        ext = np.zeros(max(ne.value, 1), dtype=np.uint32)
+        dom = np.zeros((max(ne.value, 1), W), dtype=np.uint32)
+        rc = self._lib.sky_server_result(self._h, rid, _ptr(ext, _u32p), ctypes.byref(ne),
+                                         _ptr(dom, _u32p), dom.size, None, None)
+        if rc != SKY_OK:
+            raise SkyError(rc, f"request {rid}: result copy failed")
+        return ext[:ne.value], dom[:ne.value], lat.value, bs.value
+
+    def stats(self) -> dict:
+        b, m, q = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        busy = ctypes.c_double()
+        self._lib.sky_server_stats(self._h, ctypes.byref(b), ctypes.byref(m), ctypes.byref(q),
+                                   ctypes.byref(busy))
+        return {"batches": b.value, "max_concurrent": m.value, "queued": q.value,
+                "busy_ms": busy.value}
