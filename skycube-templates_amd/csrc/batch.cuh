@@ -297,8 +297,8 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 
 // Membership, persistent: each workgroup takes (instance, block) tasks from
 // a device counter until the group's tasks (toff[T]) are done.
-template <int D, int P>
-__global__ __launch_bounds__(kBlock) void k_bmember(const float* __restrict__ erows,
+template <int D, int P, int NT>
+__global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
                                                     const uint2* __restrict__ codes,
                                                     const uint32_t* __restrict__ pos,
                                                     const uint32_t* __restrict__ eoff, uint32_t T,
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_bmember(const float* __restrict__ er
     if (task >= total) break;  // uniform
     const uint32_t t = seg_of(toff, T, task);
     const uint32_t e0 = eoff[t];
-    member_body<D, P>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
+    member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
                       (int)maxd[t], minkey + (size_t)t * 16, dom, (task - toff[t]) * P);
   }
 }

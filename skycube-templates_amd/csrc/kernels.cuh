@@ -362,6 +362,13 @@ constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
 }
 
+// Threads per workgroup of k_member: the LDS of one workgroup (two per CU
+// at d >= 13) is shared by eight waves there, four elsewhere.
+template <int D>
+constexpr int member_threads() {
+  return D >= 13 ? 512 : 256;
+}
+
 // Quantile pivots of each column, as order keys: kPivotSplit blocks per
 // column; a strided sample of up to 2048 values is ranked by counting and
 // the values at ranks b*S/16 (b = 1..15) become the pivots.  Any pivots give correct
@@ -466,9 +473,9 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
 
 // Down-closure of the P bitmaps A (W words each) in place; caller syncs
 // before.  Ends with a barrier.
-template <int D, int P, int W>
+template <int D, int P, int W, int NT>
 __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
-  for (int k = threadIdx.x; k < W; k += kBlock) {
+  for (int k = threadIdx.x; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) A[pp][k] = low_closure(A[pp][k], D);
   }
@@ -476,7 +483,7 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
   for (int j = 5; j < D; ++j) {
     __syncthreads();
     const int sh = j - 5;
-    for (int t = threadIdx.x; t < W / 2; t += kBlock) {
+    for (int t = threadIdx.x; t < W / 2; t += NT) {
       const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
 #pragma unroll
       for (int pp = 0; pp < P; ++pp) A[pp][lo] |= A[pp][lo | (1 << sh)];
@@ -515,13 +522,15 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // pair with a smaller L could, and carries its U contribution too.  So a
 // later q whose code bound on L (columns with code_q <= code_p, a superset
 // of L, from the 4-bit bit-plane codes) is set in A0 adds nothing and is
-// skipped.  Survivors of the code
-// test are compacted into an LDS queue and get the exact FP32 test together.
+// skipped.  Marks made later only add bits of the true closure, so reading
+// A0 while other waves mark it can only skip less.  Survivors of the code
+// test go to a per-wave LDS queue and get the exact FP32 test together.
 // Output: bit s = 1 iff p is not in sky(s); out-of-scope bits are 1.
 //
-// member_body processes the points base .. base+P-1 of one instance; the
-// single-instance kernel and the batched persistent kernel both call it.
-template <int D, int P>
+// member_body processes the points base .. base+P-1 of one instance with NT
+// threads; the single-instance kernel and the batched persistent kernel both
+// call it.  Candidate ids are queued as id << 3 | point, so m < 2^29.
+template <int D, int P, int NT>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
@@ -530,7 +539,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
-  constexpr int QCAP = kBlock * 2;
+  constexpr int QW = 128;              // queue entries per wave
+  constexpr int QCAP = (NT / 64) * QW;
   constexpr uint32_t FULL = (1u << D) - 1u;
   __shared__ uint32_t A0[P][W];
   __shared__ uint32_t A1[P][W];
@@ -564,7 +574,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     s_rgt[tid] = g;
     s_any[tid] = 0;
   }
-  for (int k = tid; k < W; k += kBlock) {
+  for (int k = tid; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
       A0[pp][k] = 0;
@@ -572,7 +582,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       TE[pp][k] = 0;
     }
   }
-  for (int k = tid; k < NU; k += kBlock) {
+  for (int k = tid; k < NU; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) U[pp][k] = 0;
   }
@@ -581,7 +591,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 
   // pass 1a: the lowest-score candidates, every pair tested
   const uint32_t kc = min(m, kCover);
-  for (uint32_t e = tid; e < kc * P; e += kBlock) {
+  for (uint32_t e = tid; e < kc * P; e += NT) {
     const uint32_t b = e / P;
     const int pp = (int)(e - b * P);
     uint32_t lt, eq;
@@ -589,58 +599,88 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
   }
   __syncthreads();
-  close_down<D, P, W>(A0);
+  close_down<D, P, W, NT>(A0);
 
-  // pass 1b: code test, queue, exact test of the survivors.  Each thread
-  // tests kPer candidates per round; survivors go to the LDS queue, or are
-  // tested on the spot if the queue is full.
+  // pass 1b: each thread code-tests kPer candidates against the P points
+  // per round, without branches; the rare survivors are appended to the
+  // wave's queue (tested on the spot when it is full), and a wave drains
+  // its queue when it holds 64 or more.  No workgroup barrier until the end.
   constexpr int kPer = 4;
+  static_assert(kPer * P <= 32, "survivor mask is one word");
+  const int lane = tid & 63;
+  uint32_t* wq = queue + (tid >> 6) * QW;
+  uint32_t nq = 0;  // wave-uniform
+  auto exact = [&](uint32_t b, int pp) {
+    uint32_t lt, eq;
+    pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
+    mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
+  };
+  // the queue is written and read by different lanes of one wave: LDS
+  // operations of a wave complete in order, and the fences keep the
+  // compiler from moving queue accesses across the drain
+  auto drain = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e = lane; e < nq; e += 64) {
+      const uint32_t v = wq[e];
+      exact(v >> 3, (int)(v & 7u));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    nq = 0;
+  };
   uint2 pc[P];
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
-  for (uint32_t c0 = kc; c0 < m; c0 += kBlock * kPer) {
+  constexpr uint32_t kStep = (uint32_t)NT * kPer;
+  uint2 qn[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) qn[r] = codes[min(kc + r * NT + tid, m - 1u)];
+  for (uint32_t c0 = kc; c0 < m; c0 += kStep) {
     uint2 qc[kPer];
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-      const uint32_t b = c0 + r * kBlock + tid;
-      qc[r] = b < m ? codes[b] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (int r = 0; r < kPer; ++r) qc[r] = qn[r];
+    if (c0 + kStep < m) {  // uniform: next round's codes in flight
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) qn[r] = codes[min(c0 + kStep + r * NT + tid, m - 1u)];
     }
+    uint32_t hit = 0;
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-      const uint32_t b = c0 + r * kBlock + tid;
-      if (b >= m) continue;
 #pragma unroll
       for (int pp = 0; pp < P; ++pp) {
         const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
-        if ((A0[pp][ub >> 5] >> (ub & 31)) & 1u) continue;
-        const int slot = atomicAdd(&s_cnt, 1);
-        if (slot < QCAP) {
-          queue[slot] = (b << 3) | (uint32_t)pp;
-        } else {
-          uint32_t lt, eq;
-          pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
-          mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
-        }
+        const uint32_t w = A0[pp][ub >> 5];
+        hit |= (~(w >> (ub & 31)) & 1u) << (r * P + pp);
       }
     }
-    __syncthreads();
-    const int n = min(s_cnt, QCAP);
-    if (n >= kBlock || c0 + kBlock * kPer >= m) {  // uniform
-      for (int e = tid; e < n; e += kBlock) {
-        const uint32_t v = queue[e];
-        const int pp = (int)(v & 7u);
-        uint32_t lt, eq;
-        pair_masks<D>(erows + (size_t)(v >> 3) * D, prow[pp], lt, eq);
-        mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+      if (c0 + r * NT + tid >= m) hit &= ~(((1u << P) - 1u) << (r * P));
+    if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+#pragma unroll 1
+    for (int j = 0; j < kPer * P; ++j) {
+      const uint64_t bj = __ballot((hit >> j) & 1u);
+      if (!bj) continue;
+      if ((hit >> j) & 1u) {
+        const uint32_t slot =
+            nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(bj >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bj, 0u));
+        const uint32_t b = c0 + (uint32_t)(j / P) * NT + tid;
+        if (slot < (uint32_t)QW)
+          wq[slot] = (b << 3) | (uint32_t)(j % P);
+        else
+          exact(b, j % P);
       }
-      __syncthreads();
-      if (tid == 0) s_cnt = 0;
+      nq = min(nq + (uint32_t)__popcll(bj), (uint32_t)QW);
     }
-    __syncthreads();
+    if (nq >= 64) drain();
   }
+  drain();
+  __syncthreads();
 
   // closures: A0 and A1 = A0|A1 down; TE down on R, up on Z; U superset-OR
-  for (int k = tid; k < W; k += kBlock) {
+  for (int k = tid; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
       A1[pp][k] |= A0[pp][k];
@@ -655,12 +695,12 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     }
   }
   __syncthreads();
-  close_down<D, P, W>(A0);
-  close_down<D, P, W>(A1);
+  close_down<D, P, W, NT>(A0);
+  close_down<D, P, W, NT>(A1);
 #pragma unroll 1
   for (int j = 5; j < D; ++j) {
     const int sh = j - 5;
-    for (int t = tid; t < W / 2; t += kBlock) {
+    for (int t = tid; t < W / 2; t += NT) {
       const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
       const int hi = lo | (1 << sh);
 #pragma unroll
@@ -675,7 +715,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   }
 #pragma unroll 1
   for (int j = 0; j < ZB; ++j) {
-    for (int i = tid; i < NU / 2; i += kBlock) {
+    for (int i = tid; i < NU / 2; i += NT) {
       const int lo = ((i >> j) << (j + 1)) | (i & ((1 << j) - 1));
 #pragma unroll
       for (int pp = 0; pp < P; ++pp)
@@ -697,7 +737,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       m0 |= (uint32_t)(c == 0) << b;
       m1 |= (uint32_t)(c == 1) << b;
     }
-    for (int k = tid; k < W; k += kBlock) {
+    for (int k = tid; k < W; k += NT) {
       const int ph = __popc(((uint32_t)k << 5) & ~z);
       const uint32_t r0 = ph == 0 ? m0 : 0u;
       const uint32_t r1 = ph == 0 ? m1 : (ph == 1 ? m0 : 0u);
@@ -731,7 +771,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       __syncthreads();
       if (tid == 0) s_cnt = 0;
       __syncthreads();
-      for (int k = tid; k < W; k += kBlock) {
+      for (int k = tid; k < W; k += NT) {
         uint32_t x = TE[pp][k];
         while (x) {
           const int bpos = __ffs(x) - 1;
@@ -745,7 +785,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       __syncthreads();
       const int cnt = min(s_cnt, XCAP);
       if (cnt == 0) break;
-      for (uint32_t b = tid; b < m; b += kBlock) {
+      for (uint32_t b = tid; b < m; b += NT) {
         uint32_t lt, eq;
         pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
         if (lt == 0 || (eq & ~z) == 0) continue;
@@ -763,18 +803,18 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   for (int pp = 0; pp < P; ++pp) {
     if (base + pp >= m) break;
     const uint32_t row = pos[base + pp];
-    for (int k = tid; k < W; k += kBlock) dom[(size_t)row * W + k] = A1[pp][k];
+    for (int k = tid; k < W; k += NT) dom[(size_t)row * W + k] = A1[pp][k];
   }
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
 
-template <int D, int P>
-__global__ __launch_bounds__(kBlock) void k_member(const float* __restrict__ erows,
-                                                   const uint2* __restrict__ codes,
-                                                   const uint32_t* __restrict__ pos, uint32_t m,
-                                                   int max_d, const uint32_t* __restrict__ minkey,
-                                                   uint32_t* __restrict__ dom) {
-  member_body<D, P>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P);
+template <int D, int P, int NT>
+__global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
+                                               const uint2* __restrict__ codes,
+                                               const uint32_t* __restrict__ pos, uint32_t m,
+                                               int max_d, const uint32_t* __restrict__ minkey,
+                                               uint32_t* __restrict__ dom) {
+  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P);
 }
 
 // ---------------------------------------------------------------------------

@@ -331,6 +331,8 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
   TRY(select_flagged(c, P<uint8_t>(c->flag), n, P<uint32_t>(c->ext_pids)));
   uint32_t ne = 0;
   TRY(copy_count(c, &ne));
+  if (ne >= (1u << 29))
+    return fail(c, SKY_ERR_CAPACITY, "extended skyline of %u points (limit 2^29 - 1)", ne);
   TRY(ensure(c, c->erows, sizeof(float) * (size_t)ne * d));
   TRY(ensure(c, c->dom, sizeof(uint32_t) * (size_t)ne * W));
   TRY(ensure(c, c->eidx, sizeof(uint32_t) * (size_t)ne));
@@ -370,7 +372,8 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       LAUNCHCHK();
       TRY(mark(c, SKY_STAGE_MEMBER));
       constexpr int PP = sky::member_points<D>();
-      hipLaunchKernelGGL((sky::k_member<D, PP>), dim3((ne + PP - 1) / PP), dim3(kBlock), 0,
+      constexpr int NT = sky::member_threads<D>();
+      hipLaunchKernelGGL((sky::k_member<D, PP, NT>), dim3((ne + PP - 1) / PP), dim3(NT), 0,
                          c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
                          ne, (int)max_d, minkey, P<uint32_t>(c->dom));
       LAUNCHCHK();
@@ -635,6 +638,8 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
       const uint32_t t = g.inst[k];
       if ((uint64_t)g.off[k] + n[t] > 0xFFFFFFFFull)
         return fail(c, SKY_ERR_ARG, "batch group of d=%d exceeds 2^32 points", D);
+      if (n[t] >= (1u << 29))
+        return fail(c, SKY_ERR_CAPACITY, "instance %u: %u points (limit 2^29 - 1)", t, n[t]);
       g.off[k + 1] = g.off[k] + n[t];
       sf[k] = foff[t];
       md[k] = (max_d && max_d[t] && max_d[t] < (uint32_t)D) ? max_d[t] : (uint32_t)D;
@@ -817,7 +822,8 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, bb, P<uint32_t>(B.seg_a),
                                             P<uint32_t>(B.seg_b), T + 1, c->stream));
   }
-  hipLaunchKernelGGL((sky::k_bmember<D, PP>), dim3(1024), dim3(kBlock), 0, c->stream,
+  constexpr int NT = sky::member_threads<D>();
+  hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(1024), dim3(NT), 0, c->stream,
                      P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                      P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
                      P<uint32_t>(g.dom), cnt + 4);
