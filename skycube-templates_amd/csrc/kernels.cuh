@@ -132,34 +132,19 @@ __global__ __launch_bounds__(kBlock) void k_prune(const float* __restrict__ srow
   if (valid) alive[i] = dead ? 0 : 1;
 }
 
-// Median mask of each row: bit k set iff the value is above the median
-// pivot of column k (pivot 7 of the 15 quantile pivots).  If q dominates p
-// (even weakly) then q_k <= p_k, so mask(q) is a subset of mask(p).
-template <int D>
-__global__ void k_median_masks(const float* __restrict__ rows, uint32_t m,
-                               const uint32_t* __restrict__ piv, uint32_t* __restrict__ masks) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint32_t w = 0;
-#pragma unroll
-  for (int k = 0; k < D; ++k) w |= (uint32_t)(order_key(rows[(size_t)i * D + k]) > piv[k * kCodePivots + 7]) << k;
-  masks[i] = w;
-}
-
-// Survivors against each other, one wave64 per point p: status = 2 if
-// strictly dominated, else 1 if dominated (only when need_dom), else 0.
-// The lanes walk the candidates q with key <= key(p) (score order) 64 at a
-// time; q can (weakly) dominate p only if mask(q) is a subset of mask(p),
+// Survivors against each other by a plain scan (the batched path, whose
+// instances are small), one wave64 per point p.  The lanes walk the
+// candidates q with key <= key(p) (score order) 64 at a time; q can
+// (weakly) dominate p only if mask(q) is a subset of mask(p) (median masks),
 // so only those are kept -- compacted with a ballot into a per-wave LDS
 // queue -- and get the exact FP32 test once 64 have gathered.  The wave
 // stops at the first strict dominator.  One SkyAlign pass
 // (total_pruning_moved_data_templated_pitch_two_level_full_warps,
 // src/utilities/kernels.cuh:36-411) over the whole candidate set, with its
 // partition containment test (other subset of mine) as the filter.
-// The survivor pass of point i over the candidates lo .. hi-1 (score order,
-// i inside): 2 if one strictly dominates p, else 1 if one dominates p (only
-// when need_dom), else 0.  Called by a whole wave64; q is the wave's queue
-// (192 entries of LDS).
+// Point i over the candidates lo .. hi-1 (score order, i inside): 2 if one
+// strictly dominates p, else 1 if one dominates p (only when need_dom),
+// else 0.  q is the wave's queue (192 entries of LDS).
 template <int D>
 __device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
                                              const uint32_t* __restrict__ keys,
@@ -222,17 +207,170 @@ __device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
   return st;
 }
 
+// Columns k where code_q(k) <= code_p(k), for the bit-plane codes of k_codes
+// (a lexicographic compare of the planes, 16 columns at a time): the upper
+// halves compare planes 3:2, the lower halves planes 1:0, and the upper
+// result decides unless the upper planes are equal.
+__device__ __forceinline__ uint32_t planes_le(uint2 q, uint2 p) {
+  const uint32_t le_b = ~q.y | p.y;                       // plane 2 | plane 0
+  const uint32_t le2 = (~q.x & p.x) | (~(q.x ^ p.x) & le_b);  // planes 3:2 | 1:0
+  const uint32_t ne_hi = ((q.x ^ p.x) | (q.y ^ p.y)) >> 16;   // planes 3:2 differ
+  const uint32_t hi = le2 >> 16, lo = le2 & 0xFFFFu;
+  return (ne_hi & hi) | (~ne_hi & lo);
+}
+
+// ---------------------------------------------------------------------------
+// Survivor pass over a mask index.  A weak dominator q of p has q_k <= p_k in
+// every column, so its median mask (columns above the median pivot) is a
+// subset of p's and every 4-bit code is <= p's.  The survivors are bucketed
+// by median mask (stable, so score order holds inside a bucket); point p
+// visits only the buckets of the 2^popcount(mask p) subsets of its mask, and
+// in each only the prefix with key <= key(p).
+
+// out[j] = j.
+__global__ void k_iota(uint32_t* __restrict__ out, uint32_t m) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) out[j] = j;
+}
+
+// boff[s] = first position of the mask-sorted order whose mask is >= s, for
+// s in 0 .. nb (so bucket s is boff[s] .. boff[s+1]-1).
+__global__ void k_bucket_bounds(const uint32_t* __restrict__ smask, uint32_t m, uint32_t nb,
+                                uint32_t* __restrict__ boff) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nb) return;
+  uint32_t a = 0, z = m;
+  while (a < z) {
+    const uint32_t mid = (a + z) >> 1;
+    if (smask[mid] < s) a = mid + 1;
+    else z = mid;
+  }
+  boff[s] = a;
+}
+
+// Keys and codes in bucket order.
+__global__ void k_bucket_gather(const uint32_t* __restrict__ bidx, const uint32_t* __restrict__ keys,
+                                const uint2* __restrict__ codes, uint32_t m,
+                                uint32_t* __restrict__ bkey, uint2* __restrict__ bcode) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t i = bidx[j];
+  bkey[j] = keys[i];
+  bcode[j] = codes[i];
+}
+
+// One wave64 per survivor i: status 2 if strictly dominated, else 1 if
+// dominated (only when need_dom), else 0.  Per round the lanes take 64 of
+// the subsets of mask(i), find their bucket prefixes (key <= key(i)) and
+// scan the concatenation 64 entries at a time: code test, then the exact
+// FP32 test of the few that pass.  The wave stops at a strict dominator.
+// The SkyAlign survivor pass (src/utilities/kernels.cuh:36-411) with its
+// partition containment test turned into an index: only candidates that
+// pass the containment test are visited.  sLo / sEx: 64 words each of the
+// wave's LDS.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_self_wave(const float* __restrict__ rows,
-                                                      const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ masks,
-                                                      uint32_t m, int need_dom,
-                                                      uint8_t* __restrict__ status) {
-  __shared__ uint32_t qbuf[kBlock / 64][192];
+__device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
+                                            const uint32_t* __restrict__ keys,
+                                            const uint32_t* __restrict__ masks,
+                                            const uint2* __restrict__ codes,
+                                            const uint32_t* __restrict__ boff,
+                                            const uint32_t* __restrict__ bidx,
+                                            const uint32_t* __restrict__ bkey,
+                                            const uint2* __restrict__ bcode, uint32_t i,
+                                            int need_dom, uint32_t* sLo, uint32_t* sEx) {
+  constexpr uint32_t FULL = (1u << D) - 1u;
+  const int lane = threadIdx.x & 63;
+  float p[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
+  const uint32_t kp = keys[i];
+  const uint32_t M = masks[i];
+  const uint2 pc = codes[i];
+  const uint32_t nsub = 1u << __popc(M);
+  uint8_t st = 0;
+  for (uint32_t s0 = 0; s0 < nsub && st != 2; s0 += 64) {
+    const uint32_t t = s0 + (uint32_t)lane;
+    uint32_t lo = 0, cnt = 0;
+    if (t < nsub) {
+      uint32_t S = 0, mm = M, tt = t;  // the bits of t deposited on those of M
+      while (mm) {
+        const uint32_t b = mm & (0u - mm);
+        if (tt & 1u) S |= b;
+        tt >>= 1;
+        mm &= mm - 1u;
+      }
+      lo = boff[S];
+      uint32_t a = lo, z = boff[S + 1];
+      while (a < z) {
+        const uint32_t mid = (a + z) >> 1;
+        if (bkey[mid] <= kp) a = mid + 1;
+        else z = mid;
+      }
+      cnt = a - lo;
+    }
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += v;
+    }
+    const uint32_t total = __shfl(inc, 63, 64);
+    if (total == 0) continue;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    sLo[lane] = lo;
+    sEx[lane] = inc - cnt;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+      const uint32_t e = e0 + (uint32_t)lane;
+      bool strict = false, dom = false;
+      if (e < total) {
+        int l = 0;  // the last lane whose range starts at or before e
+#pragma unroll
+        for (int step = 32; step; step >>= 1)
+          if (sEx[l + step] <= e) l += step;
+        const uint32_t j = sLo[l] + (e - sEx[l]);
+        if ((planes_le(bcode[j], pc) & FULL) == FULL) {
+          const uint32_t qi = bidx[j];
+          if (qi != i) {
+            const float* qr = rows + (size_t)qi * D;
+            uint32_t lt = 0, eq = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              const float v = qr[k];
+              lt |= (uint32_t)(v < p[k]) << k;
+              eq |= (uint32_t)(v == p[k]) << k;
+            }
+            strict = lt == FULL;
+            dom = lt != 0 && (lt | eq) == FULL;
+          }
+        }
+      }
+      if (__ballot(strict)) {
+        st = 2;
+        break;
+      }
+      if (need_dom && __ballot(dom)) st = 1;
+    }
+  }
+  return st;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_self_idx(
+    const float* __restrict__ rows, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ masks, const uint2* __restrict__ codes,
+    const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bidx,
+    const uint32_t* __restrict__ bkey, const uint2* __restrict__ bcode, uint32_t m, int need_dom,
+    uint8_t* __restrict__ status) {
+  __shared__ uint32_t sLo[kBlock / 64][64];
+  __shared__ uint32_t sEx[kBlock / 64][64];
   const int wv = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * (kBlock / 64) + wv;
   if (i >= m) return;  // whole wave; the kernel has no block barrier
-  const uint8_t st = self_scan<D>(rows, keys, masks, 0, m, i, need_dom, qbuf[wv]);
+  const uint8_t st = self_idx<D>(rows, keys, masks, codes, boff, bidx, bkey, bcode, i, need_dom,
+                                 sLo[wv], sEx[wv]);
   if ((threadIdx.x & 63) == 0) status[i] = st;
 }
 
@@ -421,18 +559,6 @@ __global__ void k_codes(const float* __restrict__ erows, uint32_t m,
   }
   codes[i] = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
   masks[i] = pl[3];
-}
-
-// Columns k where code_q(k) <= code_p(k), for the bit-plane codes above
-// (a lexicographic compare of the planes, 16 columns at a time): the upper
-// halves compare planes 3:2, the lower halves planes 1:0, and the upper
-// result decides unless the upper planes are equal.
-__device__ __forceinline__ uint32_t planes_le(uint2 q, uint2 p) {
-  const uint32_t le_b = ~q.y | p.y;                       // plane 2 | plane 0
-  const uint32_t le2 = (~q.x & p.x) | (~(q.x ^ p.x) & le_b);  // planes 3:2 | 1:0
-  const uint32_t ne_hi = ((q.x ^ p.x) | (q.y ^ p.y)) >> 16;   // planes 3:2 differ
-  const uint32_t hi = le2 >> 16, lo = le2 & 0xFFFFu;
-  return (ne_hi & hi) | (~ne_hi & lo);
 }
 
 // Marks one (q, p) pair with masks lt, eq into the structures of point pp.

@@ -66,7 +66,7 @@ struct sky_ctx {
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
       masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
-      hc_gv, hc_goff;
+      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff;
   // the last batch
   BatchState batch;
   bool has_batch = false;
@@ -218,6 +218,38 @@ int select_flagged(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out) 
 // strictly dominated) of the prefilter survivors in the context; points
 // removed by the prefilter are strictly dominated.  need_dom = 0 computes
 // only the strict part (status 0 or 2).
+// Buckets the m survivors by median mask (ds bits) for k_self_idx: bidx =
+// survivor positions in (mask, score) order, boff = bucket bounds, bkey /
+// bcode = their keys and codes in that order.
+int self_index(sky_ctx* c, uint32_t ds, uint32_t m) {
+  const uint32_t nb = 1u << ds;
+  TRY(ensure(c, c->bmask, sizeof(uint32_t) * m));
+  TRY(ensure(c, c->biota, sizeof(uint32_t) * m));
+  TRY(ensure(c, c->bidx, sizeof(uint32_t) * m));
+  TRY(ensure(c, c->bkey, sizeof(uint32_t) * m));
+  TRY(ensure(c, c->bcode, sizeof(uint2) * m));
+  TRY(ensure(c, c->boff, sizeof(uint32_t) * (nb + 1)));
+  hipLaunchKernelGGL(sky::k_iota, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->biota), m);
+  LAUNCHCHK();
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P<uint32_t>(c->masks),
+                                            P<uint32_t>(c->bmask), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), m, 0, (int)ds, c->stream));
+  TRY(ensure(c, c->cub_tmp, bytes));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, P<uint32_t>(c->masks),
+                                            P<uint32_t>(c->bmask), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), m, 0, (int)ds, c->stream));
+  hipLaunchKernelGGL(sky::k_bucket_bounds, dim3(grid_for(nb + 1)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->bmask), m, nb, P<uint32_t>(c->boff));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bucket_gather, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->bidx), P<uint32_t>(c->s1keys), P<uint2>(c->codes), m,
+                     P<uint32_t>(c->bkey), P<uint2>(c->bcode));
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
 int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, uint32_t sub,
               const uint32_t* cand_dev, uint32_t n_cand, int need_dom, uint32_t* m_out) {
   uint32_t ds = 0;
@@ -237,6 +269,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
   TRY(ensure(c, c->s1pid, sizeof(uint32_t) * nc));
   TRY(ensure(c, c->status, nc));
   TRY(ensure(c, c->masks, sizeof(uint32_t) * nc));
+  TRY(ensure(c, c->codes, sizeof(uint2) * nc));
   TRY(ensure(c, c->piv, sizeof(uint32_t) * 16 * sky::kCodePivots));
   TRY(ensure(c, c->counter, 64 * sizeof(uint32_t)));
 
@@ -286,12 +319,18 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
     hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D * sky::kPivotSplit), dim3(kBlock), 0, c->stream,
                        P<float>(c->s1rows), m, P<uint32_t>(c->piv));
     LAUNCHCHK();
-    hipLaunchKernelGGL(sky::k_median_masks<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
-                       P<float>(c->s1rows), m, P<uint32_t>(c->piv), P<uint32_t>(c->masks));
+    hipLaunchKernelGGL(sky::k_codes<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), m, P<uint32_t>(c->piv), P<uint2>(c->codes),
+                       P<uint32_t>(c->masks));
     LAUNCHCHK();
-    hipLaunchKernelGGL(sky::k_self_wave<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
-                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), P<uint32_t>(c->masks), m,
-                       need_dom, P<uint8_t>(c->status));
+  });
+  TRY(self_index(c, ds, m));
+  SKY_DISPATCH_D(ds, {
+    hipLaunchKernelGGL(sky::k_self_idx<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), P<uint32_t>(c->masks),
+                       P<uint2>(c->codes), P<uint32_t>(c->boff), P<uint32_t>(c->bidx),
+                       P<uint32_t>(c->bkey), P<uint2>(c->bcode), m, need_dom,
+                       P<uint8_t>(c->status));
     LAUNCHCHK();
   });
   *m_out = m;
@@ -887,7 +926,8 @@ void free_all(sky_ctx* c) {
                     &c->epid,   &c->pos,    &c->codes,    &c->piv,   &c->masks,
                     &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
                     &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
-                    &c->hc_goff};
+                    &c->hc_goff, &c->bmask,  &c->biota,  &c->bidx,   &c->bkey,   &c->bcode,
+                    &c->boff};
   for (DevBuf* b : bufs) free_buf(c, *b);
   free_batch_buffers(c);
   c->has_mdmc = c->has_hc = c->has_batch = false;
