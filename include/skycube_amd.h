@@ -67,6 +67,15 @@ uint32_t sky_words_per_point(uint32_t d);
  * demand and are kept for the next call. */
 size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
+/* Tuning options of a context.  SKY_OPT_MEMBER_INDEX: how the membership
+ * kernel visits candidates: -1 (default) through median-mask buckets when
+ * they hold 16 or more candidates on average, else in score order; 0 always
+ * in score order; 1 always through the buckets.  Results do not depend on
+ * it.  No reference counterpart (the reference's kernels have fixed
+ * launch shapes). */
+#define SKY_OPT_MEMBER_INDEX 1
+int sky_ctx_set_option(sky_ctx *ctx, int option, int64_t value);
+
 /* Frees every device buffer the context holds (held results included) and
  * keeps the context usable.  No reference counterpart (the reference keeps
  * its GPU state until exit). */

@@ -317,8 +317,9 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
     if (task >= total) break;  // uniform
     const uint32_t t = seg_of(toff, T, task);
     const uint32_t e0 = eoff[t];
-    member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
-                      (int)maxd[t], minkey + (size_t)t * 16, dom, (task - toff[t]) * P);
+    member_body<D, P, NT, false>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
+                      (int)maxd[t], minkey + (size_t)t * 16, dom, (task - toff[t]) * P,
+                                 nullptr, nullptr, 0u, nullptr, nullptr);
   }
 }
 

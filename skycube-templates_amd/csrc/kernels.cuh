@@ -248,15 +248,37 @@ __global__ void k_bucket_bounds(const uint32_t* __restrict__ smask, uint32_t m, 
   boff[s] = a;
 }
 
-// Keys and codes in bucket order.
+// Keys (if any) and codes in bucket order.
 __global__ void k_bucket_gather(const uint32_t* __restrict__ bidx, const uint32_t* __restrict__ keys,
                                 const uint2* __restrict__ codes, uint32_t m,
                                 uint32_t* __restrict__ bkey, uint2* __restrict__ bcode) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const uint32_t i = bidx[j];
-  bkey[j] = keys[i];
+  if (keys) bkey[j] = keys[i];
   bcode[j] = codes[i];
+}
+
+// head[j] = 1 where a run of equal sorted masks starts.
+__global__ void k_bucket_heads(const uint32_t* __restrict__ smask, uint32_t m,
+                               uint8_t* __restrict__ head) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) head[j] = j == 0 || smask[j] != smask[j - 1];
+}
+
+// The nb non-empty buckets from their head positions: mask bmk[g], start
+// bstart[g], and bstart[nb] = m.
+__global__ void k_bucket_list(const uint32_t* __restrict__ smask, const uint32_t* __restrict__ heads,
+                              uint32_t nb, uint32_t m, uint32_t* __restrict__ bmk,
+                              uint32_t* __restrict__ bstart) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > nb) return;
+  if (g == nb) {
+    bstart[g] = m;
+    return;
+  }
+  bstart[g] = heads[g];
+  bmk[g] = smask[heads[g]];
 }
 
 // One wave64 per survivor i: status 2 if strictly dominated, else 1 if
@@ -493,6 +515,10 @@ constexpr int zbits() {
   return D < 8 ? D : (D >= 15 ? 10 : 8);
 }
 constexpr uint32_t kCover = 2048;  // lowest-score candidates scanned first
+// k_member visits the candidates through the median-mask buckets when they
+// hold at least this many candidates on average (each bucket costs one
+// bound test and a share of a 64-lane round), else scans them all.
+constexpr uint32_t kMinBucket = 16;
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB.
 template <int D>
@@ -655,13 +681,19 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 //
 // member_body processes the points base .. base+P-1 of one instance with NT
 // threads; the single-instance kernel and the batched persistent kernel both
-// call it.  Candidate ids are queued as id << 3 | point, so m < 2^29.
-template <int D, int P, int NT>
+// call it.  With IDX the candidates after the cover set are visited through
+// the median-mask buckets (bmk, bstart, nbk, bidx, bcode), otherwise all in
+// score order.  Candidate ids are queued as id << 3 | point, so m < 2^29.
+template <int D, int P, int NT, bool IDX>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
                                             int max_d, const uint32_t* __restrict__ minkey,
-                                            uint32_t* __restrict__ dom, uint32_t base) {
+                                            uint32_t* __restrict__ dom, uint32_t base,
+                                            const uint32_t* __restrict__ bmk,
+                                            const uint32_t* __restrict__ bstart, uint32_t nbk,
+                                            const uint32_t* __restrict__ bidx,
+                                            const uint2* __restrict__ bcode) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
@@ -674,6 +706,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ uint32_t U[P][NU];
   __shared__ float prow[P][D];
   __shared__ uint32_t queue[QCAP];
+  __shared__ uint32_t sidx[IDX ? NT / 64 : 1][3][64];
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
@@ -727,12 +760,9 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();
   close_down<D, P, W, NT>(A0);
 
-  // pass 1b: each thread code-tests kPer candidates against the P points
-  // per round, without branches; the rare survivors are appended to the
-  // wave's queue (tested on the spot when it is full), and a wave drains
-  // its queue when it holds 64 or more.  No workgroup barrier until the end.
-  constexpr int kPer = 4;
-  static_assert(kPer * P <= 32, "survivor mask is one word");
+  // pass 1b.  Survivors of the code test (rare) are appended to the wave's
+  // queue (tested on the spot when it is full), and a wave drains its queue
+  // when it holds 64 or more.  No workgroup barrier until the end.
   const int lane = tid & 63;
   uint32_t* wq = queue + (tid >> 6) * QW;
   uint32_t nq = 0;  // wave-uniform
@@ -755,52 +785,137 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     __builtin_amdgcn_wave_barrier();
     nq = 0;
   };
+  // appends the candidate b of this lane for point pp if bit j of hit is set
+  auto append = [&](uint32_t hit, int j, int pp, uint32_t b) {
+    const uint64_t bj = __ballot((hit >> j) & 1u);
+    if (!bj) return;
+    if ((hit >> j) & 1u) {
+      const uint32_t slot =
+          nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(bj >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bj, 0u));
+      if (slot < (uint32_t)QW)
+        wq[slot] = (b << 3) | (uint32_t)pp;
+      else
+        exact(b, pp);
+    }
+    nq = min(nq + (uint32_t)__popcll(bj), (uint32_t)QW);
+  };
   uint2 pc[P];
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
-  constexpr uint32_t kStep = (uint32_t)NT * kPer;
-  uint2 qn[kPer];
+
+  if constexpr (IDX) {
+    // Candidates grouped by median mask (plane 3 of the codes): bucket g
+    // holds the candidates of mask bmk[g] at positions bstart[g] ..
+    // bstart[g+1]-1 of the bucket order.  For q in bucket g, L(q, p) lies
+    // inside FULL & ~(bmk[g] & ~mask(p)) (a column where q is above the
+    // median and p is not cannot be in L), so if that set is in A0 the
+    // whole bucket is skipped for p.  Waves take 64 buckets at a time,
+    // keep those alive for some point, and code-test their candidates 64
+    // at a time (lane e finds its bucket by a binary search of the prefix
+    // sums in LDS).
+    uint32_t* sLo = sidx[tid >> 6][0];
+    uint32_t* sEx = sidx[tid >> 6][1];
+    uint32_t* sAl = sidx[tid >> 6][2];
+    uint32_t mp[P];
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) qn[r] = codes[min(kc + r * NT + tid, m - 1u)];
-  for (uint32_t c0 = kc; c0 < m; c0 += kStep) {
-    uint2 qc[kPer];
+    for (int pp = 0; pp < P; ++pp) mp[pp] = pc[pp].x >> 16;
+    for (uint32_t g0 = (uint32_t)(tid >> 6) * 64; g0 < nbk; g0 += (uint32_t)NT) {
+      const uint32_t g = g0 + (uint32_t)lane;
+      uint32_t lo = 0, cnt = 0, al = 0;
+      if (g < nbk) {
+        const uint32_t mg = bmk[g];
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) qc[r] = qn[r];
-    if (c0 + kStep < m) {  // uniform: next round's codes in flight
-#pragma unroll
-      for (int r = 0; r < kPer; ++r) qn[r] = codes[min(c0 + kStep + r * NT + tid, m - 1u)];
-    }
-    uint32_t hit = 0;
-#pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-#pragma unroll
-      for (int pp = 0; pp < P; ++pp) {
-        const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
-        const uint32_t w = A0[pp][ub >> 5];
-        hit |= (~(w >> (ub & 31)) & 1u) << (r * P + pp);
+        for (int pp = 0; pp < P; ++pp) {
+          const uint32_t ub = FULL & ~(mg & ~mp[pp]);
+          al |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << pp;
+        }
+        if (al) {
+          lo = bstart[g];
+          cnt = bstart[g + 1] - lo;
+        }
       }
-    }
+      uint32_t inc = cnt;
 #pragma unroll
-    for (int r = 0; r < kPer; ++r)
-      if (c0 + r * NT + tid >= m) hit &= ~(((1u << P) - 1u) << (r * P));
-    if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+      }
+      const uint32_t total = __shfl(inc, 63, 64);
+      if (total == 0) continue;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      sLo[lane] = lo;
+      sEx[lane] = inc - cnt;
+      sAl[lane] = al;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+        const uint32_t e = e0 + (uint32_t)lane;
+        uint32_t hit = 0, j = 0;
+        if (e < total) {
+          int l = 0;  // the last lane whose range starts at or before e
+#pragma unroll
+          for (int step = 32; step; step >>= 1)
+            if (sEx[l + step] <= e) l += step;
+          j = sLo[l] + (e - sEx[l]);
+          const uint32_t la = sAl[l];
+          const uint2 qc = bcode[j];
+#pragma unroll
+          for (int pp = 0; pp < P; ++pp) {
+            const uint32_t ub = planes_le(qc, pc[pp]) & FULL;
+            const uint32_t w = A0[pp][ub >> 5];
+            hit |= ((~(w >> (ub & 31)) & (la >> pp)) & 1u) << pp;
+          }
+        }
+        if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+        // the cover candidates were marked in pass 1a already
+        const uint32_t b = hit ? bidx[j] : 0u;
+        if (b < kc) hit = 0;
 #pragma unroll 1
-    for (int j = 0; j < kPer * P; ++j) {
-      const uint64_t bj = __ballot((hit >> j) & 1u);
-      if (!bj) continue;
-      if ((hit >> j) & 1u) {
-        const uint32_t slot =
-            nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(bj >> 32),
-                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bj, 0u));
-        const uint32_t b = c0 + (uint32_t)(j / P) * NT + tid;
-        if (slot < (uint32_t)QW)
-          wq[slot] = (b << 3) | (uint32_t)(j % P);
-        else
-          exact(b, j % P);
+        for (int pp = 0; pp < P; ++pp) append(hit, pp, pp, b);
+        if (nq >= 64) drain();
       }
-      nq = min(nq + (uint32_t)__popcll(bj), (uint32_t)QW);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    if (nq >= 64) drain();
+  } else {
+    // Every candidate after the cover set, in score order: each thread
+    // code-tests kPer candidates against the P points per round without
+    // branches, the next round's codes already in flight.
+    constexpr int kPer = 4;
+    static_assert(kPer * P <= 32, "survivor mask is one word");
+    constexpr uint32_t kStep = (uint32_t)NT * kPer;
+    uint2 qn[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) qn[r] = codes[min(kc + r * NT + tid, m - 1u)];
+    for (uint32_t c0 = kc; c0 < m; c0 += kStep) {
+      uint2 qc[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) qc[r] = qn[r];
+      if (c0 + kStep < m) {  // uniform
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) qn[r] = codes[min(c0 + kStep + r * NT + tid, m - 1u)];
+      }
+      uint32_t hit = 0;
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+          const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
+          const uint32_t w = A0[pp][ub >> 5];
+          hit |= (~(w >> (ub & 31)) & 1u) << (r * P + pp);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r)
+        if (c0 + r * NT + tid >= m) hit &= ~(((1u << P) - 1u) << (r * P));
+      if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+#pragma unroll 1
+      for (int j = 0; j < kPer * P; ++j)
+        append(hit, j, j % P, c0 + (uint32_t)(j / P) * NT + tid);
+      if (nq >= 64) drain();
+    }
   }
   drain();
   __syncthreads();
@@ -934,13 +1049,18 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
 
-template <int D, int P, int NT>
+template <int D, int P, int NT, bool IDX>
 __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                const uint2* __restrict__ codes,
                                                const uint32_t* __restrict__ pos, uint32_t m,
                                                int max_d, const uint32_t* __restrict__ minkey,
-                                               uint32_t* __restrict__ dom) {
-  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P);
+                                               uint32_t* __restrict__ dom,
+                                               const uint32_t* __restrict__ bmk,
+                                               const uint32_t* __restrict__ bstart, uint32_t nbk,
+                                               const uint32_t* __restrict__ bidx,
+                                               const uint2* __restrict__ bcode) {
+  member_body<D, P, NT, IDX>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, bmk,
+                             bstart, nbk, bidx, bcode);
 }
 
 // ---------------------------------------------------------------------------

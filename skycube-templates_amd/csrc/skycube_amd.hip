@@ -69,6 +69,8 @@ struct sky_ctx {
       hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff;
   // the last batch
   BatchState batch;
+  // membership candidate order: -1 by bucket size, 0 scan, 1 bucket index
+  int opt_member_index = -1;
   bool has_batch = false;
   // grouping of the last HashCube export
   bool has_hc = false;
@@ -344,6 +346,46 @@ int check_ctx(sky_ctx* c) {
   return SKY_OK;
 }
 
+// Buckets the ne extended-skyline points (score order, median masks in
+// c->masks) by median mask for k_member: bidx = their positions in (mask,
+// score) order, bcode = their codes in that order, and the nbk non-empty
+// buckets: mask bkey[g], positions boff[g] .. boff[g+1]-1.
+int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
+  TRY(ensure(c, c->bmask, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->biota, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->bidx, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->bcode, sizeof(uint2) * ne));
+  TRY(ensure(c, c->bkey, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->boff, sizeof(uint32_t) * (ne + 1)));
+  TRY(ensure(c, c->alive, ne));
+  hipLaunchKernelGGL(sky::k_iota, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->biota), ne);
+  LAUNCHCHK();
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P<uint32_t>(c->masks),
+                                            P<uint32_t>(c->bmask), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), ne, 0, (int)d, c->stream));
+  TRY(ensure(c, c->cub_tmp, bytes));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, P<uint32_t>(c->masks),
+                                            P<uint32_t>(c->bmask), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), ne, 0, (int)d, c->stream));
+  hipLaunchKernelGGL(sky::k_bucket_heads, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->bmask), ne, P<uint8_t>(c->alive));
+  LAUNCHCHK();
+  // head positions into biota (free after the sort)
+  TRY(select_flagged(c, P<uint8_t>(c->alive), ne, P<uint32_t>(c->biota)));
+  TRY(copy_count(c, nbk));
+  hipLaunchKernelGGL(sky::k_bucket_list, dim3(grid_for(*nbk + 1)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->bmask), P<uint32_t>(c->biota), *nbk, ne, P<uint32_t>(c->bkey),
+                     P<uint32_t>(c->boff));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bucket_gather, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(c->bidx), nullptr, P<uint2>(c->codes), ne, nullptr,
+                     P<uint2>(c->bcode));
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
 int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
              uint32_t* n_ext) {
   c->has_mdmc = false;
@@ -410,11 +452,24 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
                          P<uint32_t>(c->masks));
       LAUNCHCHK();
       TRY(mark(c, SKY_STAGE_MEMBER));
+      uint32_t nbk = 0;
+      TRY(member_index(c, d, ne, &nbk));
       constexpr int PP = sky::member_points<D>();
       constexpr int NT = sky::member_threads<D>();
-      hipLaunchKernelGGL((sky::k_member<D, PP, NT>), dim3((ne + PP - 1) / PP), dim3(NT), 0,
-                         c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
-                         ne, (int)max_d, minkey, P<uint32_t>(c->dom));
+      const bool idx = c->opt_member_index < 0 ? ne >= (uint64_t)nbk * sky::kMinBucket
+                                                : c->opt_member_index != 0;
+      if (idx) {
+        hipLaunchKernelGGL((sky::k_member<D, PP, NT, true>), dim3((ne + PP - 1) / PP), dim3(NT),
+                           0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
+                           P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
+                           P<uint32_t>(c->bkey), P<uint32_t>(c->boff), nbk,
+                           P<uint32_t>(c->bidx), P<uint2>(c->bcode));
+      } else {
+        hipLaunchKernelGGL((sky::k_member<D, PP, NT, false>), dim3((ne + PP - 1) / PP), dim3(NT),
+                           0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
+                           P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
+                           nullptr, nullptr, 0u, nullptr, nullptr);
+      }
       LAUNCHCHK();
     });
   } else {
@@ -946,6 +1001,15 @@ void sky_ctx_destroy(sky_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
+  if (!c) return SKY_ERR_ARG;
+  if (option == SKY_OPT_MEMBER_INDEX && value >= -1 && value <= 1) {
+    c->opt_member_index = (int)value;
+    return SKY_OK;
+  }
+  return fail(c, SKY_ERR_ARG, "unknown option %d or value %lld", option, (long long)value);
 }
 
 int sky_ctx_release(sky_ctx* c) {

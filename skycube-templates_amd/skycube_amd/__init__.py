@@ -25,6 +25,7 @@ _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG_ROOT, "libskycube_amd.so")
 
 SKY_OK = 0
+SKY_OPT_MEMBER_INDEX = 1
 _ERRORS = {
     -1: "SKY_ERR_ARG", -2: "SKY_ERR_NAN", -3: "SKY_ERR_OOM_BUDGET",
     -4: "SKY_ERR_DEVICE", -5: "SKY_ERR_CAPACITY", -6: "SKY_ERR_STATE",
@@ -36,7 +37,7 @@ EXPORTS = (
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
     "sky_ctx_stage_times", "sky_mdmc_hashcube", "sky_batch_mdmc", "sky_batch_fetch",
-    "sky_ctx_release", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
+    "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
@@ -98,6 +99,7 @@ def load():
     L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     L.sky_ctx_release.argtypes = [ctypes.c_void_p]
+    L.sky_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
     L.sky_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
     L.sky_server_destroy.argtypes = [ctypes.c_void_p]
@@ -223,6 +225,14 @@ class Context:
         ms = np.zeros(len(STAGES), dtype=np.float32)
         self._check(self._L.sky_ctx_stage_times(self._h, _ptr(ms, _f32p), ms.size))
         return {k: float(v) for k, v in zip(STAGES, ms)}
+
+    def set_option(self, option: int, value: int) -> None:
+        """sky_ctx_set_option; SKY_OPT_MEMBER_INDEX = 1."""
+        self._check(self._L.sky_ctx_set_option(self._h, option, value))
+
+    def release(self) -> None:
+        """Frees the device buffers the context holds."""
+        self._check(self._L.sky_ctx_release(self._h))
 
     def bytes_in_use(self) -> int:
         return int(self._L.sky_ctx_bytes_in_use(self._h))

@@ -169,3 +169,18 @@ def test_hashcube_export(ctx, d, max_d, n):
     for s in np.flatnonzero(scope)[:200]:
         assert np.array_equal(skycube_amd.hashcube_query(gw, gv, go, ids, s),
                               np.sort(skycube_amd.members(dom, ext, s)))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("d", [3, 6, 9, 12, 14, 16])
+def test_mdmc_member_candidate_order(d, mode):
+    """Both membership candidate orders (score-order scan, median-mask
+    buckets) give the oracle's bitmaps, with ties and without."""
+    rng = np.random.default_rng(2000 + 17 * d + mode)
+    n = 3000 if d <= 12 else 1200
+    with skycube_amd.Context(0) as c:
+        c.set_option(skycube_amd.SKY_OPT_MEMBER_INDEX, mode)
+        check_mdmc(c, npref.tie_heavy(rng, n, d, 4), tag=f"order {mode} ties")
+        x = rng.random((n, d), dtype=np.float32)
+        check_mdmc(c, x, max(1, d - 3), tag=f"order {mode} continuous")
+        check_mdmc(c, datagen.anticorrelated(n, d, 50 + d), tag=f"order {mode} anticorrelated")
