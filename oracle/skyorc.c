@@ -27,9 +27,10 @@
  *   orc_mdmc       per-point closure of the pair masks over the lattice
  *                  (what the MDMC bitmap means, src/utilities/kernels.cuh:415-908)
  * plus orc_skyalign, the one-cuboid sky/ext split that run_skyalign returns
- * (src/sdsc/skyAlign.cu:9-53), and orc_score_tie_count, which finds the
- * inputs where STSC's FP32 Manhattan-norm early exit
- * (src/hybrid/mommyq.cpp:104) could make STSC differ from the definition.
+ * (src/sdsc/skyAlign.cu:9-53), and orc_score_tie_count /
+ * orc_score_tie_unresolved, which find the inputs where STSC's FP32
+ * Manhattan-norm early exit (src/hybrid/mommyq.cpp:104) could make STSC
+ * differ from the definition.
  *
  * Planted-mistake switch: orc_set_mutation() changes the pair test so the
  * tests can show that their checks catch a wrong oracle.
@@ -735,6 +736,69 @@ uint64_t orc_score_tie_count(const float *rows, int d, int max_d,
         if (!(s & lt) || __builtin_popcount(s) > max_d)
           continue;
         if (score_of(q, s, d) == score_of(p, s, d))
+          total++;
+      }
+    }
+  }
+  return total;
+}
+
+/* The triples counted by orc_score_tie_count for which p has no other
+ * dominator r in s (among the extended skyline) whose FP32 norm over s is
+ * strictly smaller than p's.  STSC scans candidates in norm order, so such a
+ * smaller-norm dominator is met before the exit and the tie is harmless; 0
+ * means STSC computes the definition on this input.  (If some dominator of p
+ * in s exists outside ext(s), one inside ext(s) strictly dominates it and has
+ * a norm no larger, so searching the extended skyline suffices.) */
+uint64_t orc_score_tie_unresolved(const float *rows, int d, int max_d,
+                                  const uint32_t *ext, uint32_t n_ext) {
+  float vmax = 0.0f;
+  for (uint32_t a = 0; a < n_ext; ++a)
+    for (int i = 0; i < d; ++i) {
+      const float v = fabsf(rows[(size_t)ext[a] * d + i]);
+      if (v > vmax)
+        vmax = v;
+    }
+  const double thr = 2.02 * (d - 1) * (double)d * (double)vmax * ldexp(1.0, -24);
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+  for (uint32_t a = 0; a < n_ext; ++a) {
+    const float *p = rows + (size_t)ext[a] * d;
+    for (uint32_t b = 0; b < n_ext; ++b) {
+      if (a == b)
+        continue;
+      const float *q = rows + (size_t)ext[b] * d;
+      uint32_t lt, eq;
+      pair_masks(q, p, d, &lt, &eq);
+      if (lt == 0)
+        continue;
+      double gap = INFINITY;
+      for (uint32_t t = lt; t; t &= t - 1) {
+        const int i = __builtin_ctz(t);
+        const double g = (double)p[i] - (double)q[i];
+        if (g < gap)
+          gap = g;
+      }
+      if (gap > thr)
+        continue;
+      const uint32_t le = lt | eq;
+      for (uint32_t s = le; s; s = (s - 1) & le) {
+        if (!(s & lt) || __builtin_popcount(s) > max_d)
+          continue;
+        const float sp = score_of(p, s, d);
+        if (score_of(q, s, d) != sp)
+          continue;
+        int found = 0;
+        for (uint32_t c = 0; c < n_ext && !found; ++c) {
+          if (c == a)
+            continue;
+          const float *r = rows + (size_t)ext[c] * d;
+          uint32_t rl, re;
+          pair_masks(r, p, d, &rl, &re);
+          if ((s & ~(rl | re)) == 0 && (s & rl) && score_of(r, s, d) < sp)
+            found = 1;
+        }
+        if (!found)
           total++;
       }
     }

@@ -42,6 +42,8 @@ def lib():
         L.orc_score_tie_count.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _u32p,
                                           ctypes.c_uint32]
         L.orc_score_tie_count.restype = ctypes.c_uint64
+        L.orc_score_tie_unresolved.argtypes = L.orc_score_tie_count.argtypes
+        L.orc_score_tie_unresolved.restype = ctypes.c_uint64
         L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_ext_skyline_timed.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_int,
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
@@ -152,6 +154,17 @@ def score_tie_count(x, ext, max_d: int | None = None) -> int:
     max_d = d if max_d is None else max_d
     ext = np.ascontiguousarray(ext, dtype=np.uint32)
     return int(lib().orc_score_tie_count(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p), ext.size))
+
+
+def score_tie_unresolved(x, ext, max_d: int | None = None) -> int:
+    """Score-tie triples with no smaller-norm dominator of p in s (the ones
+    where STSC's early exit would change its answer)."""
+    x = _rows(x)
+    n, d = x.shape
+    max_d = d if max_d is None else max_d
+    ext = np.ascontiguousarray(ext, dtype=np.uint32)
+    return int(lib().orc_score_tie_unresolved(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p),
+                                               ext.size))
 
 
 def mdmc_sample(x, ext, sample, max_d: int | None = None) -> np.ndarray:

@@ -136,9 +136,13 @@ def make_full() -> dict:
         ext, dom = skyorc.mdmc(x, w["max_d"])
         t1 = time.time() - t
         ties = skyorc.score_tie_count(x, ext, w["max_d"])
+        # triples without a smaller-norm dominator (about an hour on 8 cores
+        # for the 379 triples of the anticorrelated workload)
+        unres = skyorc.score_tie_unresolved(x, ext, w["max_d"]) if ties else 0
         sizes = sky_sizes(dom, w["d"])
         meta[wl] = {"n": w["n"], "d": w["d"], "max_d": w["max_d"], "n_ext": int(ext.size),
                     "sha256": digest(ext, dom), "score_tie_triples": ties,
+                    "score_tie_unresolved": unres,
                     "sky_sizes_sha256": hashlib.sha256(sizes.astype("<i8").tobytes()).hexdigest(),
                     "sky_size_total": int(sizes[1:].sum()), "oracle_seconds": round(t1, 1)}
         print(f"{wl}: {meta[wl]}", flush=True)

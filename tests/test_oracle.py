@@ -152,6 +152,26 @@ def test_score_tie_detector():
     assert skyorc.score_tie_count(y, skyorc.ext_skyline(y)) == 0
 
 
+def test_score_tie_resolution():
+    # q = (0.5, 2^24, 0, 1) dominates p = (1, 2^24, 1, 0) in s = {0, 1} and
+    # in s = {1, 2}, and both norms round to 2^24 in each; r = (0.25, 2^24 -
+    # 2, 0.5, 1) dominates p in both with a smaller norm, so STSC's early
+    # exit would not matter (column 3 keeps p, column 2 keeps q in the
+    # extended skyline)
+    big = np.float32(2.0 ** 24)
+    p, q = [1.0, big, 1.0, 0.0], [0.5, big, 0.0, 1.0]
+    r = [0.25, big - 2, 0.5, 1.0]
+    x = np.array([p, q], dtype=np.float32)
+    ext = skyorc.ext_skyline(x)
+    assert skyorc.score_tie_count(x, ext, 2) >= 1
+    assert skyorc.score_tie_unresolved(x, ext, 2) >= 1
+    y = np.array([p, q, r], dtype=np.float32)
+    ey = skyorc.ext_skyline(y)
+    assert ey.tolist() == [0, 1, 2]
+    assert skyorc.score_tie_count(y, ey, 2) >= 1
+    assert skyorc.score_tie_unresolved(y, ey, 2) == 0
+
+
 def test_nan_rejected():
     x = np.zeros((4, 3), dtype=np.float32)
     x[2, 1] = np.nan
