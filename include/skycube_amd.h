@@ -177,6 +177,13 @@ int sky_batch_fetch(sky_ctx *ctx, uint64_t *ext_off, uint32_t *ext_pids,
                     uint64_t *dom_off, uint32_t *dom_words, uint64_t cap_ext,
                     uint64_t cap_words);
 
+/* Results of the last batch into per-instance buffers: instance t's ids to
+ * ext_dst[t] and its bitmaps to dom_dst[t] (sized from sky_batch_fetch's
+ * offsets; a NULL pointer skips that array of the instance).  Copies go
+ * straight from device memory, at full link speed into pinned memory. */
+int sky_batch_fetch_into(sky_ctx *ctx, uint32_t *const *ext_dst,
+                         uint32_t *const *dom_dst);
+
 /*
  * One-cuboid SkyAlign: sky(s) and ext(s) \ sky(s) over a candidate set.
  * Replaces run_skyalign(full_dims, n, dimension_set, sky, ext, working_set,

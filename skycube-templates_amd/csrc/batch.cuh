@@ -236,6 +236,123 @@ __global__ __launch_bounds__(kBlock) void k_bself(const float* __restrict__ rows
   if ((threadIdx.x & 63) == 0) status[j] = st;
 }
 
+// ---------------------------------------------------------------------------
+// Mask index of a group, per instance: entries j < *count are keyed
+// (instance << D) | median mask (plane 3 of their codes) and radix-sorted
+// (stable, so score order holds inside a bucket); entries past *count get
+// all-ones keys and sort last.
+
+template <int D>
+__global__ void k_bidx_keys(const uint2* __restrict__ codes, const uint32_t* __restrict__ off,
+                            uint32_t T, const uint32_t* __restrict__ count, uint32_t N,
+                            uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  val[j] = j;
+  key[j] = j < *count ? (seg_of(off, T, j) << D) | (codes[j].x >> 16) : 0xFFFFFFFFu;
+}
+
+// boff[s] = number of sorted keys (among the first *count) below s, for
+// s = 0 .. nb: bucket (t, S) is boff[(t << D) + S] .. boff[(t << D) + S + 1].
+__global__ void k_bidx_bounds(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ count,
+                              uint32_t nb, uint32_t* __restrict__ boff) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nb) return;
+  uint32_t a = 0, z = *count;
+  while (a < z) {
+    const uint32_t mid = (a + z) >> 1;
+    if (skey[mid] < s) a = mid + 1;
+    else z = mid;
+  }
+  boff[s] = a;
+}
+
+// In bucket order, j < *count: the keys (if keys), the codes, and (if off)
+// each entry's index inside its instance.
+__global__ void k_bidx_gather(const uint32_t* __restrict__ sval, const uint32_t* __restrict__ keys,
+                              const uint2* __restrict__ codes, const uint32_t* __restrict__ count,
+                              const uint32_t* __restrict__ off, uint32_t T,
+                              uint32_t* __restrict__ bkey, uint2* __restrict__ bcode,
+                              uint32_t* __restrict__ blocal) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *count) return;
+  const uint32_t i = sval[j];
+  if (keys) bkey[j] = keys[i];
+  bcode[j] = codes[i];
+  if (off) blocal[j] = i - off[seg_of(off, T, i)];
+}
+
+// head[j] = 1 where a run of equal sorted keys starts (j < *count), 0 up to N.
+__global__ void k_bidx_heads(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ count,
+                             uint32_t N, uint8_t* __restrict__ head) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  head[j] = j < *count && (j == 0 || skey[j] != skey[j - 1]);
+}
+
+// The *nb buckets from their run heads: mask bmk[g], first position
+// bstart[g], size bcnt[g], instance brun[g].
+template <int D>
+__global__ void k_bidx_list(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ heads,
+                            const uint32_t* __restrict__ nb, const uint32_t* __restrict__ count,
+                            uint32_t N, uint32_t* __restrict__ bmk, uint32_t* __restrict__ bstart,
+                            uint32_t* __restrict__ bcnt, uint32_t* __restrict__ brun) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = *nb;
+  if (g >= n || g >= N) return;
+  const uint32_t h = heads[g];
+  const uint32_t key = skey[h];
+  bstart[g] = h;
+  bcnt[g] = (g + 1 < n ? heads[g + 1] : *count) - h;
+  bmk[g] = key & ((1u << D) - 1u);
+  brun[g] = key >> D;
+}
+
+// gfirst[t] = first bucket of instance t, t = 0 .. T (gfirst[T] = *nb).
+__global__ void k_bidx_first(const uint32_t* __restrict__ brun, const uint32_t* __restrict__ nb,
+                             uint32_t T, uint32_t* __restrict__ gfirst) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > T) return;
+  uint32_t a = 0, z = *nb;
+  while (a < z) {
+    const uint32_t mid = (a + z) >> 1;
+    if (brun[mid] < t) a = mid + 1;
+    else z = mid;
+  }
+  gfirst[t] = a;
+}
+
+// Survivor pass over the mask index, one wave64 per survivor j < *count,
+// against the survivors of its own instance (self_idx, shared with the
+// single-instance kernel; boff is the instance-major bucket table).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bself_idx(
+    const float* __restrict__ rows, const uint32_t* __restrict__ keys,
+    const uint2* __restrict__ codes, const uint32_t* __restrict__ off, uint32_t T,
+    const uint32_t* __restrict__ count, const uint32_t* __restrict__ boff,
+    const uint32_t* __restrict__ sval, const uint32_t* __restrict__ bkey,
+    const uint2* __restrict__ bcode, uint8_t* __restrict__ status) {
+  __shared__ uint32_t sLo[kBlock / 64][64];
+  __shared__ uint32_t sEx[kBlock / 64][64];
+  const int wv = threadIdx.x >> 6;
+  const uint32_t j = blockIdx.x * (kBlock / 64) + wv;
+  if (j >= *count) return;  // whole wave; no block barrier in this kernel
+  const uint32_t t = seg_of(off, T, j);
+  const uint8_t st = self_idx<D>(rows, keys, codes, boff + ((size_t)t << D), sval, bkey, bcode, j,
+                                 0, sLo[wv], sEx[wv]);
+  if ((threadIdx.x & 63) == 0) status[j] = st;
+}
+
+// Group point ids of the extended skyline as row numbers inside their
+// instance.
+__global__ void k_blocal(const uint32_t* __restrict__ ids, uint32_t cnt,
+                         const uint32_t* __restrict__ off, uint32_t T, uint32_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  const uint32_t i = ids[j];
+  out[j] = i - off[seg_of(off, T, i)];
+}
+
 // flag_pt[spid[j]] = 1 and flag_s1[j] = 1 for survivors nothing strictly
 // dominates; flag_s1 is cleared up to N.
 __global__ void k_bext_flags(const uint8_t* __restrict__ status, const uint32_t* __restrict__ spid,
@@ -296,17 +413,26 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 }
 
 // Membership, persistent: each workgroup takes (instance, block) tasks from
-// a device counter until the group's tasks (toff[T]) are done.
+// a device counter until the group's tasks (toff[T]) are done.  With gfirst,
+// instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
+// (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
+// hold kMinBucket candidates or more on average.
 template <int D, int P, int NT>
 __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
-                                                    const uint2* __restrict__ codes,
-                                                    const uint32_t* __restrict__ pos,
-                                                    const uint32_t* __restrict__ eoff, uint32_t T,
-                                                    const uint32_t* __restrict__ toff,
-                                                    const uint32_t* __restrict__ maxd,
-                                                    const uint32_t* __restrict__ minkey,
-                                                    uint32_t* __restrict__ dom,
-                                                    uint32_t* __restrict__ work) {
+                                                const uint2* __restrict__ codes,
+                                                const uint32_t* __restrict__ pos,
+                                                const uint32_t* __restrict__ eoff, uint32_t T,
+                                                const uint32_t* __restrict__ toff,
+                                                const uint32_t* __restrict__ maxd,
+                                                const uint32_t* __restrict__ minkey,
+                                                uint32_t* __restrict__ dom,
+                                                uint32_t* __restrict__ work,
+                                                const uint32_t* __restrict__ gfirst,
+                                                const uint32_t* __restrict__ bmk,
+                                                const uint32_t* __restrict__ bstart,
+                                                const uint32_t* __restrict__ bcnt,
+                                                const uint32_t* __restrict__ blocal,
+                                                const uint2* __restrict__ bcode) {
   __shared__ uint32_t s_task;
   const uint32_t total = toff[T];
   while (true) {
@@ -316,10 +442,16 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
     __syncthreads();
     if (task >= total) break;  // uniform
     const uint32_t t = seg_of(toff, T, task);
-    const uint32_t e0 = eoff[t];
-    member_body<D, P, NT, false>(erows + (size_t)e0 * D, codes + e0, pos + e0, eoff[t + 1] - e0,
-                      (int)maxd[t], minkey + (size_t)t * 16, dom, (task - toff[t]) * P,
-                                 nullptr, nullptr, 0u, nullptr, nullptr);
+    const uint32_t e0 = eoff[t], m = eoff[t + 1] - e0;
+    uint32_t g0 = 0, nb = 0;
+    if (gfirst) {
+      g0 = gfirst[t];
+      nb = gfirst[t + 1] - g0;
+    }
+    const bool use_idx = gfirst && m >= nb * kMinBucket;
+    member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, m, (int)maxd[t],
+                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, use_idx,
+                          bmk + g0, bstart + g0, bcnt + g0, nb, blocal, bcode);
   }
 }
 

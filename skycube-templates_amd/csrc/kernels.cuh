@@ -266,18 +266,15 @@ __global__ void k_bucket_heads(const uint32_t* __restrict__ smask, uint32_t m,
   if (j < m) head[j] = j == 0 || smask[j] != smask[j - 1];
 }
 
-// The nb non-empty buckets from their head positions: mask bmk[g], start
-// bstart[g], and bstart[nb] = m.
+// The nb non-empty buckets from their head positions: mask bmk[g], first
+// position bstart[g], size bcnt[g].
 __global__ void k_bucket_list(const uint32_t* __restrict__ smask, const uint32_t* __restrict__ heads,
                               uint32_t nb, uint32_t m, uint32_t* __restrict__ bmk,
-                              uint32_t* __restrict__ bstart) {
+                              uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcnt) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g > nb) return;
-  if (g == nb) {
-    bstart[g] = m;
-    return;
-  }
+  if (g >= nb) return;
   bstart[g] = heads[g];
+  bcnt[g] = (g + 1 < nb ? heads[g + 1] : m) - heads[g];
   bmk[g] = smask[heads[g]];
 }
 
@@ -293,7 +290,6 @@ __global__ void k_bucket_list(const uint32_t* __restrict__ smask, const uint32_t
 template <int D>
 __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
                                             const uint32_t* __restrict__ keys,
-                                            const uint32_t* __restrict__ masks,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ boff,
                                             const uint32_t* __restrict__ bidx,
@@ -306,8 +302,8 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
 #pragma unroll
   for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
   const uint32_t kp = keys[i];
-  const uint32_t M = masks[i];
   const uint2 pc = codes[i];
+  const uint32_t M = pc.x >> 16;  // plane 3: the median mask
   const uint32_t nsub = 1u << __popc(M);
   uint8_t st = 0;
   for (uint32_t s0 = 0; s0 < nsub && st != 2; s0 += 64) {
@@ -382,7 +378,7 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_self_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
-    const uint32_t* __restrict__ masks, const uint2* __restrict__ codes,
+    const uint2* __restrict__ codes,
     const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bidx,
     const uint32_t* __restrict__ bkey, const uint2* __restrict__ bcode, uint32_t m, int need_dom,
     uint8_t* __restrict__ status) {
@@ -391,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_self_idx(
   const int wv = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * (kBlock / 64) + wv;
   if (i >= m) return;  // whole wave; the kernel has no block barrier
-  const uint8_t st = self_idx<D>(rows, keys, masks, codes, boff, bidx, bkey, bcode, i, need_dom,
+  const uint8_t st = self_idx<D>(rows, keys, codes, boff, bidx, bkey, bcode, i, need_dom,
                                  sLo[wv], sEx[wv]);
   if ((threadIdx.x & 63) == 0) status[i] = st;
 }
@@ -681,17 +677,20 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 //
 // member_body processes the points base .. base+P-1 of one instance with NT
 // threads; the single-instance kernel and the batched persistent kernel both
-// call it.  With IDX the candidates after the cover set are visited through
-// the median-mask buckets (bmk, bstart, nbk, bidx, bcode), otherwise all in
-// score order.  Candidate ids are queued as id << 3 | point, so m < 2^29.
-template <int D, int P, int NT, bool IDX>
+// call it.  With use_idx (uniform) the candidates after the cover set are
+// visited through the median-mask buckets (bucket g: mask bmk[g], positions
+// bstart[g] .. bstart[g] + bcnt[g] - 1 of the bucket order, whose codes are
+// bcode and candidate ids bidx), otherwise all in score order.  Candidate
+// ids are queued as id << 3 | point, so m < 2^29.
+template <int D, int P, int NT>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
                                             int max_d, const uint32_t* __restrict__ minkey,
                                             uint32_t* __restrict__ dom, uint32_t base,
-                                            const uint32_t* __restrict__ bmk,
-                                            const uint32_t* __restrict__ bstart, uint32_t nbk,
+                                            bool use_idx, const uint32_t* __restrict__ bmk,
+                                            const uint32_t* __restrict__ bstart,
+                                            const uint32_t* __restrict__ bcnt, uint32_t nbk,
                                             const uint32_t* __restrict__ bidx,
                                             const uint2* __restrict__ bcode) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
@@ -706,7 +705,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ uint32_t U[P][NU];
   __shared__ float prow[P][D];
   __shared__ uint32_t queue[QCAP];
-  __shared__ uint32_t sidx[IDX ? NT / 64 : 1][3][64];
+  __shared__ uint32_t sidx[NT / 64][3][64];
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
@@ -804,10 +803,10 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
 
-  if constexpr (IDX) {
+  if (use_idx) {
     // Candidates grouped by median mask (plane 3 of the codes): bucket g
     // holds the candidates of mask bmk[g] at positions bstart[g] ..
-    // bstart[g+1]-1 of the bucket order.  For q in bucket g, L(q, p) lies
+    // bstart[g] + bcnt[g] - 1 of the bucket order.  For q in bucket g, L(q, p) lies
     // inside FULL & ~(bmk[g] & ~mask(p)) (a column where q is above the
     // median and p is not cannot be in L), so if that set is in A0 the
     // whole bucket is skipped for p.  Waves take 64 buckets at a time,
@@ -832,7 +831,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         }
         if (al) {
           lo = bstart[g];
-          cnt = bstart[g + 1] - lo;
+          cnt = bcnt[g];
         }
       }
       uint32_t inc = cnt;
@@ -1056,11 +1055,12 @@ __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                int max_d, const uint32_t* __restrict__ minkey,
                                                uint32_t* __restrict__ dom,
                                                const uint32_t* __restrict__ bmk,
-                                               const uint32_t* __restrict__ bstart, uint32_t nbk,
+                                               const uint32_t* __restrict__ bstart,
+                                               const uint32_t* __restrict__ bcnt, uint32_t nbk,
                                                const uint32_t* __restrict__ bidx,
                                                const uint2* __restrict__ bcode) {
-  member_body<D, P, NT, IDX>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, bmk,
-                             bstart, nbk, bidx, bcode);
+  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, IDX, bmk,
+                        bstart, bcnt, nbk, bidx, bcode);
 }
 
 // ---------------------------------------------------------------------------

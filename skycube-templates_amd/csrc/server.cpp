@@ -9,9 +9,11 @@
 // worker's host work (row upload, set-up, result download) overlaps the
 // other's kernels.  Each worker owns hbm_budget / workers bytes of device
 // memory; a batch is admitted only while the estimated footprint of its
-// requests fits that share.  Results are copied to host memory owned by the
-// request until sky_server_result hands them over.  The reference has no
-// counterpart (it runs one skycube per process).
+// requests fits that share.  Rows go up from a pinned staging buffer per
+// worker; results come down straight from the batch's device buffers into
+// pinned host memory owned by the request (an arena of pinned slabs) until
+// sky_server_result copies them out.  The reference has no counterpart (it
+// runs one skycube per process).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +37,59 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Pinned host memory for results: slabs of at least kSlab bytes, each a bump
+// allocator that starts over once every region in it has been released.
+// Past kMaxPinned bytes of slabs, results fall back to pageable memory.
+class PinnedArena {
+ public:
+  static constexpr size_t kSlab = 256ull << 20;
+  static constexpr size_t kMaxPinned = 64ull << 30;
+  ~PinnedArena() {
+    for (Slab& sl : slabs_) (void)hipHostFree(sl.p);
+  }
+  // Region of `bytes` (256-byte aligned); *slab = its slab, or -1 if none.
+  void* alloc(size_t bytes, int* slab) {
+    bytes = (bytes + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> lk(mu_);
+    for (size_t i = 0; i < slabs_.size(); ++i) {
+      Slab& sl = slabs_[i];
+      if (sl.cap - sl.top >= bytes) {
+        void* p = sl.p + sl.top;
+        sl.top += bytes;
+        sl.live++;
+        *slab = (int)i;
+        return p;
+      }
+    }
+    const size_t cap = std::max(kSlab, bytes);
+    void* p = nullptr;
+    if (total_ + cap > kMaxPinned || hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess) {
+      *slab = -1;
+      return nullptr;
+    }
+    total_ += cap;
+    slabs_.push_back({static_cast<uint8_t*>(p), cap, bytes, 1});
+    *slab = (int)slabs_.size() - 1;
+    return p;
+  }
+  void release(int slab) {
+    if (slab < 0) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    Slab& sl = slabs_[(size_t)slab];
+    if (--sl.live == 0) sl.top = 0;
+  }
+
+ private:
+  struct Slab {
+    uint8_t* p;
+    size_t cap, top;
+    uint32_t live;
+  };
+  std::mutex mu_;
+  std::vector<Slab> slabs_;
+  size_t total_ = 0;
+};
+
 struct Request {
   uint64_t id = 0;
   std::vector<float> rows;
@@ -45,7 +100,36 @@ struct Request {
   int rc = SKY_OK;
   std::string err;
   uint32_t batch_size = 0;
-  std::vector<uint32_t> ext, dom;
+  // result: n_ext ids and n_words bitmap words, in a pinned region (slab
+  // >= 0) or in the fallback vectors
+  uint32_t n_ext = 0;
+  uint64_t n_words = 0;
+  uint32_t *ext_p = nullptr, *dom_p = nullptr;
+  int slab = -1;
+  std::vector<uint32_t> ext_v, dom_v;
+};
+
+// Pinned staging buffer for the rows of one worker's batch.
+struct Staging {
+  float* p = nullptr;
+  size_t cap = 0;
+  ~Staging() {
+    if (p) (void)hipHostFree(p);
+  }
+  float* get(size_t floats) {
+    if (floats > cap) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      void* q = nullptr;
+      const size_t want = std::max<size_t>(floats + floats / 4, 1u << 20);
+      if (hipHostMalloc(&q, want * sizeof(float), hipHostMallocDefault) != hipSuccess)
+        return nullptr;
+      p = static_cast<float*>(q);
+      cap = want;
+    }
+    return p;
+  }
 };
 
 }  // namespace
@@ -60,6 +144,7 @@ struct sky_server {
   bool stop = false;
   std::vector<sky_ctx*> ctxs;
   std::vector<std::thread> workers;
+  PinnedArena arena;
   uint64_t batches = 0, max_concurrent = 0, concurrent = 0;
   double busy_s = 0, busy_since = 0;  // time with at least one batch running
 };
@@ -67,12 +152,13 @@ struct sky_server {
 extern "C" uint64_t sky_server_footprint(uint32_t n, uint32_t d) {
   // Device bytes of one instance in a batched pass, an upper bound: shared
   // scratch per point (four 16-float row copies, 64-bit sort keys in two
-  // buffers, ids, flags, codes: 332 B), sort temporaries (16 B), the row
-  // upload (4d B), and ids and bitmaps as if every point were in the
-  // extended skyline; per instance, pivots, minima and offsets (1.1 KB).
-  // Buffers are allocated with 1/8 headroom.
-  const uint64_t per_point = 332 + 16 + 4ull * d + 4 + 4ull * sky_words_per_point(d);
-  return ((uint64_t)n * per_point + 1100) * 9 / 8;
+  // buffers, ids, flags, codes: 332 B; the mask index: 52 B), sort
+  // temporaries (16 B), the row upload (4d B), and ids and bitmaps as if
+  // every point were in the extended skyline; per instance, pivots, minima,
+  // offsets (1.1 KB) and the bucket table (4 B per median mask).  Buffers
+  // are allocated with 1/8 headroom.
+  const uint64_t per_point = 332 + 52 + 16 + 4ull * d + 4 + 4ull * sky_words_per_point(d);
+  return ((uint64_t)n * per_point + 1100 + (4ull << d)) * 9 / 8;
 }
 
 namespace {
@@ -81,10 +167,12 @@ namespace {
 // offsets of the sixteen dimensionality groups, sort temporaries).
 constexpr uint64_t kBatchFixed = 4ull << 20;
 
-void run_batch(sky_server* s, sky_ctx* ctx, std::vector<Request*>& take, uint64_t need) {
-  std::vector<uint32_t> n(take.size()), d(take.size()), m(take.size());
+void run_batch(sky_server* s, sky_ctx* ctx, Staging& stage, std::vector<Request*>& take,
+               uint64_t need) {
+  const size_t B = take.size();
+  std::vector<uint32_t> n(B), d(B), m(B);
   size_t total = 0;
-  for (size_t k = 0; k < take.size(); ++k) {
+  for (size_t k = 0; k < B; ++k) {
     n[k] = take[k]->n;
     d[k] = take[k]->d;
     m[k] = take[k]->max_d;
@@ -92,44 +180,62 @@ void run_batch(sky_server* s, sky_ctx* ctx, std::vector<Request*>& take, uint64_
   }
   // buffers kept from earlier batches count against the share: drop them
   // when they and this batch together might not fit
-  int rc0 = SKY_OK;
-  if (sky_ctx_bytes_in_use(ctx) + need > s->share) rc0 = sky_ctx_release(ctx);
-  std::vector<float> rows(std::max<size_t>(total, 1));
+  int rc = SKY_OK;
+  if (sky_ctx_bytes_in_use(ctx) + need > s->share) rc = sky_ctx_release(ctx);
+  std::vector<float> pageable;
+  float* rows = stage.get(std::max<size_t>(total, 1));
+  if (!rows) {
+    pageable.resize(std::max<size_t>(total, 1));
+    rows = pageable.data();
+  }
   size_t o = 0;
   for (Request* r : take) {
-    std::copy(r->rows.begin(), r->rows.end(), rows.begin() + (long)o);
+    std::copy(r->rows.begin(), r->rows.end(), rows + o);
     o += r->rows.size();
   }
   uint64_t tot = 0;
-  int rc = rc0;
   if (rc == SKY_OK)
-    rc = sky_batch_mdmc(ctx, rows.data(), 0, (uint32_t)take.size(), n.data(), d.data(), m.data(),
-                        0, &tot);
-  std::vector<uint64_t> eo(take.size() + 1), dof(take.size() + 1);
-  std::vector<uint32_t> ids, words;
+    rc = sky_batch_mdmc(ctx, rows, 0, (uint32_t)B, n.data(), d.data(), m.data(), 0, &tot);
+  std::vector<uint64_t> eo(B + 1), dof(B + 1);
   if (rc == SKY_OK) rc = sky_batch_fetch(ctx, eo.data(), nullptr, dof.data(), nullptr, 0, 0);
   if (rc == SKY_OK) {
-    ids.resize(std::max<uint64_t>(eo.back(), 1));
-    words.resize(std::max<uint64_t>(dof.back(), 1));
-    rc = sky_batch_fetch(ctx, eo.data(), ids.data(), dof.data(), words.data(), ids.size(),
-                         words.size());
+    std::vector<uint32_t*> ed(B), dd(B);
+    for (size_t k = 0; k < B; ++k) {
+      Request* r = take[k];
+      r->n_ext = (uint32_t)(eo[k + 1] - eo[k]);
+      r->n_words = dof[k + 1] - dof[k];
+      const size_t ext_bytes = ((size_t)r->n_ext * 4 + 255) & ~size_t(255);
+      const size_t bytes = ext_bytes + (size_t)r->n_words * 4;
+      uint8_t* p = bytes ? static_cast<uint8_t*>(s->arena.alloc(bytes, &r->slab)) : nullptr;
+      if (bytes && !p) {  // no pinned memory left: pageable
+        r->ext_v.resize(r->n_ext);
+        r->dom_v.resize(r->n_words);
+        r->ext_p = r->ext_v.data();
+        r->dom_p = r->dom_v.data();
+      } else if (p) {
+        r->ext_p = reinterpret_cast<uint32_t*>(p);
+        r->dom_p = reinterpret_cast<uint32_t*>(p + ext_bytes);
+      }
+      ed[k] = r->n_ext ? r->ext_p : nullptr;
+      dd[k] = r->n_words ? r->dom_p : nullptr;
+    }
+    rc = sky_batch_fetch_into(ctx, ed.data(), dd.data());
   }
   const std::string err = rc == SKY_OK ? "" : sky_last_error(ctx);
   const double t = now_s();
   std::lock_guard<std::mutex> lk(s->mu);
-  for (size_t k = 0; k < take.size(); ++k) {
-    Request* r = take[k];
+  for (Request* r : take) {
     r->rc = rc;
     r->err = err;
-    if (rc == SKY_OK) {
-      r->ext.assign(ids.begin() + (long)eo[k], ids.begin() + (long)eo[k + 1]);
-      r->dom.assign(words.begin() + (long)dof[k], words.begin() + (long)dof[k + 1]);
-    }
     r->rows.clear();
     r->rows.shrink_to_fit();
     r->t_done = t;
     r->state = 2;
-    r->batch_size = (uint32_t)take.size();
+    r->batch_size = (uint32_t)B;
+    if (rc != SKY_OK) {
+      s->arena.release(r->slab);
+      r->slab = -1;
+    }
   }
   s->in_flight -= need;
   s->batches++;
@@ -140,15 +246,21 @@ void run_batch(sky_server* s, sky_ctx* ctx, std::vector<Request*>& take, uint64_
 
 void worker(sky_server* s, size_t w) {
   sky_ctx* ctx = s->ctxs[w];
+  Staging stage;
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
     if (s->stop && s->queue.empty()) return;
     // admit, first come first served, while the footprint fits this
-    // worker's share; every request fits a share alone (checked at submit)
+    // worker's share (every request fits a share alone, checked at submit),
+    // and at most an equal split of the queue over the workers, so that a
+    // burst keeps every worker's batch in the pipeline
     std::vector<Request*> take;
     uint64_t need = kBatchFixed;
-    while (!s->queue.empty() && take.size() < s->max_batch) {
+    const size_t nw = s->ctxs.size();
+    const size_t cap =
+        std::min<size_t>(s->max_batch, std::max<size_t>(1, (s->queue.size() + nw - 1) / nw));
+    while (!s->queue.empty() && take.size() < cap) {
       Request* r = s->queue.front();
       if (!take.empty() && need + r->footprint > s->share) break;
       need += r->footprint;
@@ -161,7 +273,7 @@ void worker(sky_server* s, size_t w) {
     if (s->concurrent++ == 0) s->busy_since = now_s();
     s->max_concurrent = std::max(s->max_concurrent, s->concurrent);
     lk.unlock();
-    run_batch(s, ctx, take, need);
+    run_batch(s, ctx, stage, take, need);
     lk.lock();
   }
 }
@@ -256,24 +368,29 @@ int sky_server_result(sky_server* s, uint64_t id, uint32_t* ext_pids, uint32_t* 
                       uint32_t* dom_words, uint64_t cap_words, double* latency_ms,
                       uint32_t* batch_size) {
   if (!s || !n_ext) return SKY_ERR_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->all.find(id);
-  if (it == s->all.end()) return SKY_ERR_STATE;
-  Request* r = it->second.get();
-  if (r->state != 2) return SKY_ERR_STATE;
-  if (r->rc != SKY_OK) {
-    const int rc = r->rc;
+  std::unique_ptr<Request> own;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return SKY_ERR_STATE;
+    Request* r = it->second.get();
+    if (r->state != 2) return SKY_ERR_STATE;
+    if (r->rc != SKY_OK) {
+      const int rc = r->rc;
+      s->all.erase(it);
+      return rc;
+    }
+    *n_ext = r->n_ext;
+    if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+    if (batch_size) *batch_size = r->batch_size;
+    if (!ext_pids || !dom_words) return SKY_OK;  // sizes only; the result stays
+    if (cap_words < r->n_words) return SKY_ERR_CAPACITY;
+    own = std::move(it->second);  // the copy runs outside the lock
     s->all.erase(it);
-    return rc;
   }
-  *n_ext = (uint32_t)r->ext.size();
-  if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
-  if (batch_size) *batch_size = r->batch_size;
-  if (!ext_pids || !dom_words) return SKY_OK;  // sizes only; the result stays
-  if (cap_words < r->dom.size()) return SKY_ERR_CAPACITY;
-  std::copy(r->ext.begin(), r->ext.end(), ext_pids);
-  std::copy(r->dom.begin(), r->dom.end(), dom_words);
-  s->all.erase(it);
+  if (own->n_ext) std::memcpy(ext_pids, own->ext_p, sizeof(uint32_t) * own->n_ext);
+  if (own->n_words) std::memcpy(dom_words, own->dom_p, sizeof(uint32_t) * own->n_words);
+  s->arena.release(own->slab);
   return SKY_OK;
 }
 

@@ -36,6 +36,7 @@ struct BatchGroup {
   uint32_t T = 0, N = 0, ntasks = 0;
   DevBuf d_off, d_src_foff, d_tasks, d_maxd;  // descriptors
   DevBuf ext_pids, eoff, dom, cnt;            // results; cnt: counters
+  mutable std::vector<uint32_t> eo_host;       // eoff read back by the last fetch
 };
 
 struct BatchState {
@@ -51,6 +52,9 @@ struct BatchState {
   DevBuf rows, key_a, key_b, val_a, val_b, srows, skeys, spid, alive, s1idx, s1rows, s1keys,
       s1pid, masks, status, flag_pt, flag_s1, eidx, erows, epid, pos, codes, rank, piv, minkey,
       seg_a, seg_b, tmp;
+  // mask index: sort keys / values, bucket order, bucket list, bucket table
+  DevBuf ikey, ikey2, ival, isort, ibkey, ibcode, iblocal, ihead, ibmk, ibstart, ibcnt, ibrun,
+      igfirst, iboff;
   size_t tmp_bytes = 0;
 };
 
@@ -66,7 +70,7 @@ struct sky_ctx {
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
       masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
-      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff;
+      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff, bcnt;
   // the last batch
   BatchState batch;
   // membership candidate order: -1 by bucket size, 0 scan, 1 bucket index
@@ -329,7 +333,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
   TRY(self_index(c, ds, m));
   SKY_DISPATCH_D(ds, {
     hipLaunchKernelGGL(sky::k_self_idx<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
-                       P<float>(c->s1rows), P<uint32_t>(c->s1keys), P<uint32_t>(c->masks),
+                       P<float>(c->s1rows), P<uint32_t>(c->s1keys),
                        P<uint2>(c->codes), P<uint32_t>(c->boff), P<uint32_t>(c->bidx),
                        P<uint32_t>(c->bkey), P<uint2>(c->bcode), m, need_dom,
                        P<uint8_t>(c->status));
@@ -349,7 +353,7 @@ int check_ctx(sky_ctx* c) {
 // Buckets the ne extended-skyline points (score order, median masks in
 // c->masks) by median mask for k_member: bidx = their positions in (mask,
 // score) order, bcode = their codes in that order, and the nbk non-empty
-// buckets: mask bkey[g], positions boff[g] .. boff[g+1]-1.
+// buckets: mask bkey[g], positions boff[g] .. boff[g] + bcnt[g] - 1.
 int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
   TRY(ensure(c, c->bmask, sizeof(uint32_t) * ne));
   TRY(ensure(c, c->biota, sizeof(uint32_t) * ne));
@@ -357,6 +361,7 @@ int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
   TRY(ensure(c, c->bcode, sizeof(uint2) * ne));
   TRY(ensure(c, c->bkey, sizeof(uint32_t) * ne));
   TRY(ensure(c, c->boff, sizeof(uint32_t) * (ne + 1)));
+  TRY(ensure(c, c->bcnt, sizeof(uint32_t) * ne));
   TRY(ensure(c, c->alive, ne));
   hipLaunchKernelGGL(sky::k_iota, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
                      P<uint32_t>(c->biota), ne);
@@ -375,9 +380,9 @@ int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
   // head positions into biota (free after the sort)
   TRY(select_flagged(c, P<uint8_t>(c->alive), ne, P<uint32_t>(c->biota)));
   TRY(copy_count(c, nbk));
-  hipLaunchKernelGGL(sky::k_bucket_list, dim3(grid_for(*nbk + 1)), dim3(kBlock), 0, c->stream,
+  hipLaunchKernelGGL(sky::k_bucket_list, dim3(grid_for(*nbk)), dim3(kBlock), 0, c->stream,
                      P<uint32_t>(c->bmask), P<uint32_t>(c->biota), *nbk, ne, P<uint32_t>(c->bkey),
-                     P<uint32_t>(c->boff));
+                     P<uint32_t>(c->boff), P<uint32_t>(c->bcnt));
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_bucket_gather, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
                      P<uint32_t>(c->bidx), nullptr, P<uint2>(c->codes), ne, nullptr,
@@ -462,13 +467,13 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, true>), dim3((ne + PP - 1) / PP), dim3(NT),
                            0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
                            P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           P<uint32_t>(c->bkey), P<uint32_t>(c->boff), nbk,
-                           P<uint32_t>(c->bidx), P<uint2>(c->bcode));
+                           P<uint32_t>(c->bkey), P<uint32_t>(c->boff), P<uint32_t>(c->bcnt),
+                           nbk, P<uint32_t>(c->bidx), P<uint2>(c->bcode));
       } else {
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, false>), dim3((ne + PP - 1) / PP), dim3(NT),
                            0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
                            P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           nullptr, nullptr, 0u, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, 0u, nullptr, nullptr);
       }
       LAUNCHCHK();
     });
@@ -660,7 +665,9 @@ void free_batch_buffers(sky_ctx* c) {
   DevBuf* sb[] = {&B.rows,  &B.key_a,  &B.key_b,  &B.val_a, &B.val_b,  &B.srows,  &B.skeys,
                   &B.spid,  &B.alive,  &B.s1idx,  &B.s1rows, &B.s1keys, &B.s1pid, &B.masks,
                   &B.status, &B.flag_pt, &B.flag_s1, &B.eidx, &B.erows, &B.epid,  &B.pos,
-                  &B.codes, &B.rank,   &B.piv,    &B.minkey, &B.seg_a, &B.seg_b, &B.tmp};
+                  &B.codes, &B.rank,   &B.piv,    &B.minkey, &B.seg_a, &B.seg_b, &B.tmp,
+                  &B.ikey,  &B.ikey2,  &B.ival,   &B.isort,  &B.ibkey, &B.ibcode, &B.iblocal,
+                  &B.ihead, &B.ibmk,   &B.ibstart, &B.ibcnt, &B.ibrun, &B.igfirst, &B.iboff};
   for (DevBuf* b : sb) free_buf(c, *b);
   B.tmp_bytes = 0;
 }
@@ -683,12 +690,20 @@ int fill8(sky_ctx* c, void* p, uint64_t n, uint8_t v) {
   return SKY_OK;
 }
 
+// The batched survivor pass uses the mask index when its bucket table (one
+// entry per instance and median mask) stays within 2^26 entries.
+bool batch_self_index(int D, uint32_t T) { return ((uint64_t)T << D) <= (1ull << 26); }
+
 // Temp bytes of every rocPRIM call of a group of N points, T instances.
 int batch_tmp_bytes(sky_ctx* c, uint32_t N, uint32_t T, size_t* out) {
   size_t mx = 0, b = 0;
   hipcub::CountingInputIterator<uint32_t> it(0);
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                             (uint32_t*)nullptr, (uint32_t*)nullptr, N, 0, 64,
+                                            c->stream));
+  mx = std::max(mx, b);
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                            (uint32_t*)nullptr, (uint32_t*)nullptr, N, 0, 32,
                                             c->stream));
   mx = std::max(mx, b);
   HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b, it, (uint8_t*)nullptr, (uint32_t*)nullptr,
@@ -790,6 +805,16 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   TRY(ensure(c, B.minkey, sizeof(uint32_t) * T * 16));
   TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
   TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
+  DevBuf* ib[] = {&B.ikey, &B.ikey2, &B.ival, &B.isort, &B.ibkey, &B.iblocal, &B.ibmk, &B.ibstart,
+                  &B.ibcnt, &B.ibrun};
+  for (DevBuf* b : ib) TRY(ensure(c, *b, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.ibcode, sizeof(uint2) * N));
+  TRY(ensure(c, B.ihead, N));
+  TRY(ensure(c, B.igfirst, sizeof(uint32_t) * (T + 1)));
+  uint64_t tab = 1;
+  for (const BatchGroup* g : B.groups)
+    if (batch_self_index(g->D, g->T)) tab = std::max<uint64_t>(tab, ((uint64_t)g->T << g->D) + 1);
+  TRY(ensure(c, B.iboff, sizeof(uint32_t) * tab));
   size_t tb = 0;
   for (const BatchGroup* g : B.groups) {
     size_t b = 0;
@@ -827,6 +852,24 @@ int select_dev(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out, uint
   hipcub::CountingInputIterator<uint32_t> it(0);
   size_t b = c->batch.tmp_bytes;
   HIPCHK(hipcub::DeviceSelect::Flagged(c->batch.tmp.p, b, it, flags, out, count, n, c->stream));
+  return SKY_OK;
+}
+
+// Keys (instance << D | median mask) of the entries j < *count of a group
+// (segments off) sorted with their positions: ikey2 / isort.
+template <int D>
+int batch_mask_sort(sky_ctx* c, const uint32_t* off, uint32_t T, uint32_t N,
+                    const uint32_t* count) {
+  BatchState& B = c->batch;
+  hipLaunchKernelGGL(sky::k_bidx_keys<D>, dim3(grid_for(N)), dim3(kBlock), 0, c->stream,
+                     P<uint2>(B.codes), off, T, count, N, P<uint32_t>(B.ikey), P<uint32_t>(B.ival));
+  LAUNCHCHK();
+  int tbits = 0;
+  while ((1u << tbits) < T) ++tbits;
+  size_t b = B.tmp_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, b, P<uint32_t>(B.ikey), P<uint32_t>(B.ikey2),
+                                            P<uint32_t>(B.ival), P<uint32_t>(B.isort), N, 0,
+                                            std::min(32, tbits + D), c->stream));
   return SKY_OK;
 }
 
@@ -872,13 +915,35 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
                      P<float>(B.s1rows), s1off, T, P<uint32_t>(B.piv));
   LAUNCHCHK();
-  hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.s1rows),
-                     s1off, T, cnt + 1, P<uint32_t>(B.piv), P<uint32_t>(B.masks));
-  LAUNCHCHK();
-  hipLaunchKernelGGL(sky::k_bself<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
-                     P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint32_t>(B.masks), s1off, T,
-                     cnt + 1, P<uint8_t>(B.status));
-  LAUNCHCHK();
+  if (batch_self_index(D, T)) {
+    // survivors' codes, then the instance-major mask index and its table
+    hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), s1off, T, cnt + 1, P<uint32_t>(B.piv),
+                       P<uint2>(B.codes));
+    LAUNCHCHK();
+    TRY(batch_mask_sort<D>(c, s1off, T, N, cnt + 1));
+    hipLaunchKernelGGL(sky::k_bidx_bounds, dim3(grid_for(((uint64_t)T << D) + 1)), dim3(kBlock), 0,
+                       c->stream, P<uint32_t>(B.ikey2), cnt + 1, T << D, P<uint32_t>(B.iboff));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bidx_gather, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.isort), P<uint32_t>(B.s1keys), P<uint2>(B.codes), cnt + 1,
+                       nullptr, T, P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes), s1off, T,
+                       cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort), P<uint32_t>(B.ibkey),
+                       P<uint2>(B.ibcode), P<uint8_t>(B.status));
+    LAUNCHCHK();
+  } else {
+    hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), s1off, T, cnt + 1, P<uint32_t>(B.piv),
+                       P<uint32_t>(B.masks));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bself<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint32_t>(B.masks), s1off, T,
+                       cnt + 1, P<uint8_t>(B.status));
+    LAUNCHCHK();
+  }
   hipLaunchKernelGGL(sky::k_bext_flags, dim3(gN), dim3(kBlock), 0, c->stream,
                      P<uint8_t>(B.status), P<uint32_t>(B.s1pid), cnt + 1, N,
                      P<uint8_t>(B.flag_pt), P<uint8_t>(B.flag_s1));
@@ -907,6 +972,24 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.erows),
                      eoff, T, cnt + 3, P<uint32_t>(B.piv), P<uint2>(B.codes));
   LAUNCHCHK();
+  // bucket list of the extended skyline per instance (count in cnt[5])
+  TRY(batch_mask_sort<D>(c, eoff, T, N, cnt + 3));
+  hipLaunchKernelGGL(sky::k_bidx_heads, dim3(gN), dim3(kBlock), 0, c->stream, P<uint32_t>(B.ikey2),
+                     cnt + 3, N, P<uint8_t>(B.ihead));
+  LAUNCHCHK();
+  TRY(select_dev(c, P<uint8_t>(B.ihead), N, P<uint32_t>(B.ival), cnt + 5));
+  hipLaunchKernelGGL(sky::k_bidx_list<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(B.ikey2), P<uint32_t>(B.ival), cnt + 5, cnt + 3, N,
+                     P<uint32_t>(B.ibmk), P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt),
+                     P<uint32_t>(B.ibrun));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bidx_first, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(B.ibrun), cnt + 5, T, P<uint32_t>(B.igfirst));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bidx_gather, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint32_t>(B.isort), nullptr, P<uint2>(B.codes), cnt + 3, eoff, T, nullptr,
+                     P<uint2>(B.ibcode), P<uint32_t>(B.iblocal));
+  LAUNCHCHK();
   constexpr int PP = sky::member_points<D>();
   hipLaunchKernelGGL(sky::k_btasks, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream, eoff, T,
                      (uint32_t)PP, P<uint32_t>(B.seg_a));
@@ -920,7 +1003,9 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(1024), dim3(NT), 0, c->stream,
                      P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                      P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                     P<uint32_t>(g.dom), cnt + 4);
+                     P<uint32_t>(g.dom), cnt + 4, P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
+                     P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
+                     P<uint2>(B.ibcode));
   LAUNCHCHK();
   return SKY_OK;
 }
@@ -931,6 +1016,60 @@ int batch_enqueue(sky_ctx* c, const float* rows_dev) {
     SKY_DISPATCH_D(g->D, { rc = batch_group<D>(c, *g, rows_dev); });
     if (rc != SKY_OK) return rc;
   }
+  return SKY_OK;
+}
+
+// Extended-skyline size and dimensionality of every instance of the last
+// batch (group result offsets read back from the device).
+int batch_sizes(sky_ctx* c, std::vector<uint32_t>& ne, std::vector<uint32_t>& dim) {
+  BatchState& B = c->batch;
+  ne.assign(B.n_inst, 0);
+  dim.assign(B.sig_d.begin(), B.sig_d.end());
+  std::vector<std::vector<uint32_t>> eo(B.groups.size());
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    eo[gi].resize(B.groups[gi]->T + 1);
+    HIPCHK(hipMemcpyAsync(eo[gi].data(), B.groups[gi]->eoff.p,
+                          sizeof(uint32_t) * (B.groups[gi]->T + 1), hipMemcpyDeviceToHost,
+                          c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    const BatchGroup& g = *B.groups[gi];
+    g.eo_host.assign(eo[gi].begin(), eo[gi].end());
+    for (uint32_t k = 0; k < g.T; ++k) ne[g.inst[k]] = eo[gi][k + 1] - eo[gi][k];
+  }
+  return SKY_OK;
+}
+
+// Copies instance t's extended-skyline ids (local row numbers) to ext_dst[t]
+// and its bitmaps to dom_dst[t] (null pointers skip an instance), straight
+// from the group buffers; ids are made local on the device first.
+int batch_copy_out(sky_ctx* c, uint32_t* const* ext_dst, uint32_t* const* dom_dst) {
+  BatchState& B = c->batch;
+  std::vector<uint32_t> ne, dim;
+  TRY(batch_sizes(c, ne, dim));
+  for (BatchGroup* gp : B.groups) {
+    BatchGroup& g = *gp;
+    const uint32_t tot = g.eo_host[g.T];
+    if (!tot) continue;
+    const uint32_t W = sky_words_per_point(g.D);
+    uint32_t* loc = P<uint32_t>(B.isort);  // scratch of the last pass
+    hipLaunchKernelGGL(sky::k_blocal, dim3(grid_for(tot)), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(g.ext_pids), tot, P<uint32_t>(g.d_off), g.T, loc);
+    LAUNCHCHK();
+    for (uint32_t k = 0; k < g.T; ++k) {
+      const uint32_t t = g.inst[k], e0 = g.eo_host[k], n_e = g.eo_host[k + 1] - e0;
+      if (!n_e) continue;
+      if (ext_dst[t])
+        HIPCHK(hipMemcpyAsync(ext_dst[t], loc + e0, sizeof(uint32_t) * n_e,
+                              hipMemcpyDeviceToHost, c->stream));
+      if (dom_dst[t])
+        HIPCHK(hipMemcpyAsync(dom_dst[t], P<uint32_t>(g.dom) + (size_t)e0 * W,
+                              sizeof(uint32_t) * (size_t)n_e * W, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return SKY_OK;
 }
 
@@ -982,7 +1121,7 @@ void free_all(sky_ctx* c) {
                     &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
                     &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
                     &c->hc_goff, &c->bmask,  &c->biota,  &c->bidx,   &c->bkey,   &c->bcode,
-                    &c->boff};
+                    &c->boff,    &c->bcnt};
   for (DevBuf* b : bufs) free_buf(c, *b);
   free_batch_buffers(c);
   c->has_mdmc = c->has_hc = c->has_batch = false;
@@ -1178,25 +1317,9 @@ int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t*
   TRY(check_ctx(c));
   if (!ext_off || !dom_off) return fail(c, SKY_ERR_ARG, "null argument");
   if (!c->has_batch) return fail(c, SKY_ERR_STATE, "no batch result is held");
-  BatchState& B = c->batch;
-  const uint32_t NI = B.n_inst;
-  std::vector<uint32_t> ne(NI, 0), dim(NI, 0);
-  std::vector<std::vector<uint32_t>> eo(B.groups.size());
-  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
-    eo[gi].resize(B.groups[gi]->T + 1);
-    HIPCHK(hipMemcpyAsync(eo[gi].data(), B.groups[gi]->eoff.p,
-                          sizeof(uint32_t) * (B.groups[gi]->T + 1), hipMemcpyDeviceToHost,
-                          c->stream));
-  }
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
-    const BatchGroup& g = *B.groups[gi];
-    for (uint32_t k = 0; k < g.T; ++k) {
-      ne[g.inst[k]] = eo[gi][k + 1] - eo[gi][k];
-      dim[g.inst[k]] = (uint32_t)g.D;
-    }
-  }
-  for (uint32_t t = 0; t < NI; ++t) dim[t] = dim[t] ? dim[t] : B.sig_d[t];
+  const uint32_t NI = c->batch.n_inst;
+  std::vector<uint32_t> ne, dim;
+  TRY(batch_sizes(c, ne, dim));
   ext_off[0] = dom_off[0] = 0;
   for (uint32_t t = 0; t < NI; ++t) {
     ext_off[t + 1] = ext_off[t] + ne[t];
@@ -1206,27 +1329,19 @@ int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t*
   if (cap_ext < ext_off[NI] || cap_words < dom_off[NI])
     return fail(c, SKY_ERR_CAPACITY, "batch needs %llu ids and %llu words",
                 (unsigned long long)ext_off[NI], (unsigned long long)dom_off[NI]);
-  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
-    const BatchGroup& g = *B.groups[gi];
-    const uint32_t tot = eo[gi][g.T];
-    const uint32_t W = sky_words_per_point(g.D);
-    std::vector<uint32_t> ids(tot), words((size_t)tot * W);
-    if (tot) {
-      HIPCHK(hipMemcpyAsync(ids.data(), g.ext_pids.p, sizeof(uint32_t) * tot,
-                            hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(words.data(), g.dom.p, sizeof(uint32_t) * words.size(),
-                            hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-    }
-    for (uint32_t k = 0; k < g.T; ++k) {
-      const uint32_t t = g.inst[k];
-      for (uint32_t j = eo[gi][k]; j < eo[gi][k + 1]; ++j)
-        ext_pids[ext_off[t] + (j - eo[gi][k])] = ids[j] - g.off[k];
-      std::copy(words.begin() + (size_t)eo[gi][k] * W, words.begin() + (size_t)eo[gi][k + 1] * W,
-                dom_words + dom_off[t]);
-    }
+  std::vector<uint32_t*> ed(NI), dd(NI);
+  for (uint32_t t = 0; t < NI; ++t) {
+    ed[t] = ext_pids + ext_off[t];
+    dd[t] = dom_words + dom_off[t];
   }
-  return SKY_OK;
+  return batch_copy_out(c, ed.data(), dd.data());
+}
+
+int sky_batch_fetch_into(sky_ctx* c, uint32_t* const* ext_dst, uint32_t* const* dom_dst) {
+  TRY(check_ctx(c));
+  if (!ext_dst || !dom_dst) return fail(c, SKY_ERR_ARG, "null argument");
+  if (!c->has_batch) return fail(c, SKY_ERR_STATE, "no batch result is held");
+  return batch_copy_out(c, ext_dst, dom_dst);
 }
 
 int sky_skyalign(sky_ctx* c, const float* rows, uint32_t n_rows, uint32_t d, uint32_t subspace,

@@ -37,6 +37,7 @@ EXPORTS = (
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
     "sky_ctx_stage_times", "sky_mdmc_hashcube", "sky_batch_mdmc", "sky_batch_fetch",
+    "sky_batch_fetch_into",
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
 )
@@ -96,6 +97,8 @@ def load():
                                   _u32p, _u32p, _u32p, ctypes.c_int, _u64p]
     L.sky_batch_fetch.argtypes = [ctypes.c_void_p, _u64p, _u32p, _u64p, _u32p, ctypes.c_uint64,
                                   ctypes.c_uint64]
+    L.sky_batch_fetch_into.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p)]
     L.sky_ctx_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     L.sky_ctx_release.argtypes = [ctypes.c_void_p]
