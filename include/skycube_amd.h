@@ -67,14 +67,32 @@ uint32_t sky_words_per_point(uint32_t d);
  * demand and are kept for the next call. */
 size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
-/* Tuning options of a context.  SKY_OPT_MEMBER_INDEX: how the membership
- * kernel visits candidates: -1 (default) through median-mask buckets when
- * they hold 16 or more candidates on average, else in score order; 0 always
- * in score order; 1 always through the buckets.  Results do not depend on
- * it.  No reference counterpart (the reference's kernels have fixed
- * launch shapes). */
+/* Tuning options of a context (results never depend on them).  No
+ * reference counterpart (the reference's kernels have fixed launch shapes).
+ *   SKY_OPT_MEMBER_INDEX  how the membership kernel visits candidates: -1
+ *                         (default) through median-mask buckets when they
+ *                         hold SKY_OPT_MIN_BUCKET or more candidates on
+ *                         average, else in score order; 0 always in score
+ *                         order; 1 always through the buckets
+ *   SKY_OPT_COVER         candidates tested exactly before the coverage
+ *                         pruning starts (default 2048; 64 .. 65536)
+ *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (1024; 64 .. 8192)
+ *   SKY_OPT_MIN_BUCKET    bucket-size threshold above (16; 1 .. 2^30)
+ *   SKY_OPT_MEMBER_GRID   workgroups of the batched persistent membership
+ *                         kernel (1024; 64 .. 8192)
+ *   SKY_OPT_TILE          points per membership workgroup in the batched
+ *                         path: 0 the default, 1 half of it
+ * option | (d << 8) sets the value for dimensionality d only (d = 1..16);
+ * a plain option sets it for every d.  sky_ctx_get_option reads one back
+ * (for d = 1 when no d is given). */
 #define SKY_OPT_MEMBER_INDEX 1
+#define SKY_OPT_COVER 2
+#define SKY_OPT_PRUNERS 3
+#define SKY_OPT_MIN_BUCKET 4
+#define SKY_OPT_MEMBER_GRID 5
+#define SKY_OPT_TILE 6
 int sky_ctx_set_option(sky_ctx *ctx, int option, int64_t value);
+int sky_ctx_get_option(sky_ctx *ctx, int option, int64_t *value);
 
 /* Frees every device buffer the context holds (held results included) and
  * keeps the context usable.  No reference counterpart (the reference keeps
@@ -176,6 +194,18 @@ int sky_batch_mdmc(sky_ctx *ctx, const float *rows, int rows_on_device,
 int sky_batch_fetch(sky_ctx *ctx, uint64_t *ext_off, uint32_t *ext_pids,
                     uint64_t *dom_off, uint32_t *dom_words, uint64_t cap_ext,
                     uint64_t cap_words);
+
+/* Autotuner of the batched path: runs the batch (as sky_batch_mdmc would)
+ * once per candidate value of each setting in turn -- tile (SKY_OPT_TILE),
+ * cover depth, prefilter size, bucket threshold, persistent grid -- timing
+ * every dimensionality group with HIP events, and keeps for each group's d
+ * the fastest value (coordinate descent, about 19 passes; tune on a sample
+ * of a large batch).  The chosen settings stay on the context; report (if
+ * not NULL) receives them as JSON text.  The last batch's results are not
+ * kept.  No reference counterpart. */
+int sky_batch_autotune(sky_ctx *ctx, const float *rows, int rows_on_device,
+                       uint32_t n_inst, const uint32_t *n, const uint32_t *d,
+                       const uint32_t *max_d, char *report, size_t report_cap);
 
 /* Results of the last batch into per-instance buffers: instance t's ids to
  * ext_dst[t] and its bitmaps to dom_dst[t] (sized from sky_batch_fetch's

@@ -90,12 +90,13 @@ __global__ void k_bgather(const float* __restrict__ rows, const uint64_t* __rest
 }
 
 // Prefilter, one workgroup per task (instance t, first position): the
-// points of the task against the instance's kPruners lowest-norm points.
+// points of the task against the instance's npr_max lowest-norm points.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_bprune(const float* __restrict__ srows,
                                                    const uint32_t* __restrict__ skeys,
                                                    const uint32_t* __restrict__ off,
                                                    const uint2* __restrict__ tasks,
+                                                   uint32_t npr_max,
                                                    uint8_t* __restrict__ alive) {
   __shared__ float tile[kBlock * D];
   __shared__ uint32_t tkey[kBlock];
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_bprune(const float* __restrict__ sro
     for (int k = 0; k < D; ++k) p[k] = srows[(size_t)i * D + k];
     kp = skeys[i];
   }
-  const uint32_t npr = min(nt, kPruners);
+  const uint32_t npr = min(nt, npr_max);
   bool dead = false;
   for (uint32_t t0 = 0; t0 < npr; t0 += kBlock) {
     const uint32_t cnt = min((uint32_t)kBlock, npr - t0);
@@ -416,7 +417,7 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // a device counter until the group's tasks (toff[T]) are done.  With gfirst,
 // instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
 // (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
-// hold kMinBucket candidates or more on average.
+// hold min_bucket candidates or more on average; cover as in member_body.
 template <int D, int P, int NT>
 __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
                                                 const uint2* __restrict__ codes,
@@ -426,7 +427,8 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
                                                 const uint32_t* __restrict__ maxd,
                                                 const uint32_t* __restrict__ minkey,
                                                 uint32_t* __restrict__ dom,
-                                                uint32_t* __restrict__ work,
+                                                uint32_t* __restrict__ work, uint32_t cover,
+                                                uint32_t min_bucket,
                                                 const uint32_t* __restrict__ gfirst,
                                                 const uint32_t* __restrict__ bmk,
                                                 const uint32_t* __restrict__ bstart,
@@ -448,9 +450,9 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
       g0 = gfirst[t];
       nb = gfirst[t + 1] - g0;
     }
-    const bool use_idx = gfirst && m >= nb * kMinBucket;
+    const bool use_idx = gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
     member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, m, (int)maxd[t],
-                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, use_idx,
+                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover, use_idx,
                           bmk + g0, bstart + g0, bcnt + g0, nb, blocal, bcode);
   }
 }

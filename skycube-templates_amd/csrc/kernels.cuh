@@ -17,7 +17,7 @@
 namespace sky {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
-constexpr uint32_t kPruners = 1024;  // lowest-score points used as a prefilter
+constexpr uint32_t kPruners = 1024;  // default: lowest-score points used as a prefilter
 constexpr int kXCap = 2048;       // tie-touched subspaces resolved per round
 constexpr int kCodePivots = 15;   // 16 quantile buckets per column, 4-bit codes
 
@@ -510,16 +510,22 @@ template <int D>
 constexpr int zbits() {
   return D < 8 ? D : (D >= 15 ? 10 : 8);
 }
-constexpr uint32_t kCover = 2048;  // lowest-score candidates scanned first
-// k_member visits the candidates through the median-mask buckets when they
-// hold at least this many candidates on average (each bucket costs one
-// bound test and a share of a 64-lane round), else scans them all.
+constexpr uint32_t kCover = 2048;  // default: lowest-score candidates tested first
+// Default: k_member visits the candidates through the median-mask buckets
+// when they hold at least this many candidates on average (each bucket
+// costs one bound test and a share of a 64-lane round), else scans them.
 constexpr uint32_t kMinBucket = 16;
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB.
 template <int D>
 constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
+}
+
+// The other tile width the batch autotuner may pick: half as many points.
+template <int D>
+constexpr int member_points_alt() {
+  return member_points<D>() > 1 ? member_points<D>() / 2 : 1;
 }
 
 // Threads per workgroup of k_member: the LDS of one workgroup (two per CU
@@ -664,7 +670,7 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 //                        the rest, X, is decided by a second pass over the
 //                        pairs with residual ties (rare).
 //
-// Pruning: the kCover lowest-score candidates are marked first and A0 is
+// Pruning: the `cover` lowest-score candidates are marked first and A0 is
 // closed.  Any set bit of A0 is then inside the L of a marked pair with no
 // residual tie, which dominates p in every subspace (sR non-empty) that a
 // pair with a smaller L could, and carries its U contribution too.  So a
@@ -688,7 +694,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint32_t* __restrict__ pos, uint32_t m,
                                             int max_d, const uint32_t* __restrict__ minkey,
                                             uint32_t* __restrict__ dom, uint32_t base,
-                                            bool use_idx, const uint32_t* __restrict__ bmk,
+                                            uint32_t cover, bool use_idx,
+                                            const uint32_t* __restrict__ bmk,
                                             const uint32_t* __restrict__ bstart,
                                             const uint32_t* __restrict__ bcnt, uint32_t nbk,
                                             const uint32_t* __restrict__ bidx,
@@ -748,7 +755,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();
 
   // pass 1a: the lowest-score candidates, every pair tested
-  const uint32_t kc = min(m, kCover);
+  const uint32_t kc = min(m, cover);
   for (uint32_t e = tid; e < kc * P; e += NT) {
     const uint32_t b = e / P;
     const int pp = (int)(e - b * P);
@@ -1053,14 +1060,14 @@ __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                const uint2* __restrict__ codes,
                                                const uint32_t* __restrict__ pos, uint32_t m,
                                                int max_d, const uint32_t* __restrict__ minkey,
-                                               uint32_t* __restrict__ dom,
+                                               uint32_t* __restrict__ dom, uint32_t cover,
                                                const uint32_t* __restrict__ bmk,
                                                const uint32_t* __restrict__ bstart,
                                                const uint32_t* __restrict__ bcnt, uint32_t nbk,
                                                const uint32_t* __restrict__ bidx,
                                                const uint2* __restrict__ bcode) {
-  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, IDX, bmk,
-                        bstart, bcnt, nbk, bidx, bcode);
+  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover, IDX,
+                        bmk, bstart, bcnt, nbk, bidx, bcode);
 }
 
 // ---------------------------------------------------------------------------

@@ -48,6 +48,7 @@ struct BatchState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool ready = false;
+  uint64_t sig_tune = 0;
   // shared scratch, sized for the largest group
   DevBuf rows, key_a, key_b, val_a, val_b, srows, skeys, spid, alive, s1idx, s1rows, s1keys,
       s1pid, masks, status, flag_pt, flag_s1, eidx, erows, epid, pos, codes, rank, piv, minkey,
@@ -56,6 +57,15 @@ struct BatchState {
   DevBuf ikey, ikey2, ival, isort, ibkey, ibcode, iblocal, ihead, ibmk, ibstart, ibcnt, ibrun,
       igfirst, iboff;
   size_t tmp_bytes = 0;
+};
+
+// Launch settings of one dimensionality (sky_ctx_set_option, autotuner).
+struct Tuning {
+  uint32_t cover = sky::kCover;           // candidates tested exactly first
+  uint32_t pruners = sky::kPruners;       // prefilter points
+  uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
+  uint32_t grid = 1024;                   // workgroups of the persistent kernel
+  uint32_t tile = 0;                      // 0: member_points, 1: half of it
 };
 
 }  // namespace
@@ -75,6 +85,8 @@ struct sky_ctx {
   BatchState batch;
   // membership candidate order: -1 by bucket size, 0 scan, 1 bucket index
   int opt_member_index = -1;
+  Tuning tune[17];       // per dimensionality
+  uint64_t tune_ver = 0;  // bumped on every change (cached graphs are rebuilt)
   bool has_batch = false;
   // grouping of the last HashCube export
   bool has_hc = false;
@@ -301,7 +313,7 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
                                             P<uint32_t>(c->key_b), P<uint32_t>(c->pid_a),
                                             P<uint32_t>(c->pid_b), nc, 0, 32, c->stream));
 
-  const uint32_t npr = std::min<uint32_t>(nc, sky::kPruners);
+  const uint32_t npr = std::min<uint32_t>(nc, c->tune[ds].pruners);
   SKY_DISPATCH_D(ds, {
     hipLaunchKernelGGL(sky::k_gather<D>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream,
                        rows_dev, d, P<uint32_t>(c->pid_b), nc, dims, P<float>(c->srows));
@@ -461,19 +473,20 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       TRY(member_index(c, d, ne, &nbk));
       constexpr int PP = sky::member_points<D>();
       constexpr int NT = sky::member_threads<D>();
-      const bool idx = c->opt_member_index < 0 ? ne >= (uint64_t)nbk * sky::kMinBucket
+      const Tuning& tu = c->tune[d];
+      const bool idx = c->opt_member_index < 0 ? ne >= (uint64_t)nbk * tu.min_bucket
                                                 : c->opt_member_index != 0;
       if (idx) {
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, true>), dim3((ne + PP - 1) / PP), dim3(NT),
                            0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
                            P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           P<uint32_t>(c->bkey), P<uint32_t>(c->boff), P<uint32_t>(c->bcnt),
+                           tu.cover, P<uint32_t>(c->bkey), P<uint32_t>(c->boff), P<uint32_t>(c->bcnt),
                            nbk, P<uint32_t>(c->bidx), P<uint2>(c->bcode));
       } else {
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, false>), dim3((ne + PP - 1) / PP), dim3(NT),
                            0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
                            P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           nullptr, nullptr, nullptr, 0u, nullptr, nullptr);
+                           tu.cover, nullptr, nullptr, nullptr, 0u, nullptr, nullptr);
       }
       LAUNCHCHK();
     });
@@ -824,6 +837,7 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   TRY(ensure(c, B.tmp, tb));
   B.tmp_bytes = tb;
   B.sig_rows = rows_dev;
+  B.sig_tune = c->tune_ver;
   B.sig_n.assign(n, n + n_inst);
   B.sig_d.assign(d, d + n_inst);
   B.sig_m.assign(n_inst, 0);
@@ -873,6 +887,32 @@ int batch_mask_sort(sky_ctx* c, const uint32_t* off, uint32_t T, uint32_t N,
   return SKY_OK;
 }
 
+// The membership stage of a group: task numbering for tiles of PP points,
+// then the persistent kernel.
+template <int D, int PP>
+int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& tu) {
+  BatchState& B = c->batch;
+  const uint32_t T = g.T;
+  uint32_t* cnt = P<uint32_t>(g.cnt);
+  hipLaunchKernelGGL(sky::k_btasks, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream, eoff, T,
+                     (uint32_t)PP, P<uint32_t>(B.seg_a));
+  LAUNCHCHK();
+  {
+    size_t bb = B.tmp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, bb, P<uint32_t>(B.seg_a),
+                                            P<uint32_t>(B.seg_b), T + 1, c->stream));
+  }
+  constexpr int NT = sky::member_threads<D>();
+  hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
+                     P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
+                     P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
+                     P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.min_bucket,
+                     P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk), P<uint32_t>(B.ibstart),
+                     P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal), P<uint2>(B.ibcode));
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
 // Enqueues one group's whole pass on the context stream (no host sync, no
 // allocation, no memset: graph-capturable).
 template <int D>
@@ -900,9 +940,10 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
                      P<uint64_t>(B.key_b), P<uint32_t>(B.val_b), N, P<float>(B.srows),
                      P<uint32_t>(B.skeys), P<uint32_t>(B.spid));
   LAUNCHCHK();
+  const Tuning& tu = c->tune[D];
   hipLaunchKernelGGL(sky::k_bprune<D>, dim3(g.ntasks), dim3(kBlock), 0, c->stream,
                      P<float>(B.srows), P<uint32_t>(B.skeys), off, P<uint2>(g.d_tasks),
-                     P<uint8_t>(B.alive));
+                     tu.pruners, P<uint8_t>(B.alive));
   LAUNCHCHK();
   TRY(select_dev(c, P<uint8_t>(B.alive), N, P<uint32_t>(B.s1idx), cnt + 1));
   uint32_t* s1off = P<uint32_t>(B.seg_b);
@@ -990,23 +1031,10 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
                      P<uint32_t>(B.isort), nullptr, P<uint2>(B.codes), cnt + 3, eoff, T, nullptr,
                      P<uint2>(B.ibcode), P<uint32_t>(B.iblocal));
   LAUNCHCHK();
-  constexpr int PP = sky::member_points<D>();
-  hipLaunchKernelGGL(sky::k_btasks, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream, eoff, T,
-                     (uint32_t)PP, P<uint32_t>(B.seg_a));
-  LAUNCHCHK();
-  {
-    size_t bb = B.tmp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, bb, P<uint32_t>(B.seg_a),
-                                            P<uint32_t>(B.seg_b), T + 1, c->stream));
-  }
-  constexpr int NT = sky::member_threads<D>();
-  hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(1024), dim3(NT), 0, c->stream,
-                     P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
-                     P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                     P<uint32_t>(g.dom), cnt + 4, P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
-                     P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
-                     P<uint2>(B.ibcode));
-  LAUNCHCHK();
+  if (tu.tile == 0)
+    TRY((batch_member<D, sky::member_points<D>()>(c, g, eoff, tu)));
+  else
+    TRY((batch_member<D, sky::member_points_alt<D>()>(c, g, eoff, tu)));
   return SKY_OK;
 }
 
@@ -1144,11 +1172,130 @@ void sky_ctx_destroy(sky_ctx* c) {
 
 int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
   if (!c) return SKY_ERR_ARG;
-  if (option == SKY_OPT_MEMBER_INDEX && value >= -1 && value <= 1) {
+  const int opt = option & 0xFF, dd = option >> 8;
+  if (dd < 0 || dd > 16) return fail(c, SKY_ERR_ARG, "option %d: dimensionality %d", option, dd);
+  if (opt == SKY_OPT_MEMBER_INDEX) {
+    if (value < -1 || value > 1 || dd) return fail(c, SKY_ERR_ARG, "bad member index option");
     c->opt_member_index = (int)value;
     return SKY_OK;
   }
-  return fail(c, SKY_ERR_ARG, "unknown option %d or value %lld", option, (long long)value);
+  struct Range {
+    int opt;
+    int64_t lo, hi;
+    uint32_t Tuning::*field;
+  };
+  static const Range ranges[] = {
+      {SKY_OPT_COVER, 64, 1 << 16, &Tuning::cover},
+      {SKY_OPT_PRUNERS, 64, 1 << 13, &Tuning::pruners},
+      {SKY_OPT_MIN_BUCKET, 1, 1 << 30, &Tuning::min_bucket},
+      {SKY_OPT_MEMBER_GRID, 64, 1 << 13, &Tuning::grid},
+      {SKY_OPT_TILE, 0, 1, &Tuning::tile},
+  };
+  for (const Range& r : ranges) {
+    if (r.opt != opt) continue;
+    if (value < r.lo || value > r.hi)
+      return fail(c, SKY_ERR_ARG, "option %d: value %lld outside [%lld, %lld]", opt,
+                  (long long)value, (long long)r.lo, (long long)r.hi);
+    for (int k = dd ? dd : 1; k <= (dd ? dd : 16); ++k) c->tune[k].*(r.field) = (uint32_t)value;
+    c->tune_ver++;
+    return SKY_OK;
+  }
+  return fail(c, SKY_ERR_ARG, "unknown option %d", option);
+}
+
+int sky_ctx_get_option(sky_ctx* c, int option, int64_t* value) {
+  if (!c || !value) return SKY_ERR_ARG;
+  const int opt = option & 0xFF, dd = option >> 8;
+  if (dd < 0 || dd > 16) return SKY_ERR_ARG;
+  const Tuning& t = c->tune[dd ? dd : 1];
+  switch (opt) {
+    case SKY_OPT_MEMBER_INDEX: *value = c->opt_member_index; return SKY_OK;
+    case SKY_OPT_COVER: *value = t.cover; return SKY_OK;
+    case SKY_OPT_PRUNERS: *value = t.pruners; return SKY_OK;
+    case SKY_OPT_MIN_BUCKET: *value = t.min_bucket; return SKY_OK;
+    case SKY_OPT_MEMBER_GRID: *value = t.grid; return SKY_OK;
+    case SKY_OPT_TILE: *value = t.tile; return SKY_OK;
+    default: return fail(c, SKY_ERR_ARG, "unknown option %d", option);
+  }
+}
+
+int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n_inst,
+                       const uint32_t* n, const uint32_t* d, const uint32_t* max_d, char* report,
+                       size_t report_cap) {
+  TRY(check_ctx(c));
+  if (!n || !d || (n_inst && !rows)) return fail(c, SKY_ERR_ARG, "null argument");
+  uint64_t tot = 0;
+  // a first pass builds the batch and loads every kernel
+  TRY(sky_batch_mdmc(c, rows, rows_on_device, n_inst, n, d, max_d, 0, &tot));
+  const float* rows_dev = rows_on_device ? rows : P<float>(c->rows);
+  BatchState& B = c->batch;
+  const size_t G = B.groups.size();
+  std::vector<hipEvent_t> ev(2 * G, nullptr);
+  for (hipEvent_t& e : ev) HIPCHK(hipEventCreate(&e));
+  struct Knob {
+    const char* name;
+    uint32_t Tuning::*field;
+    std::vector<uint32_t> values;
+  };
+  const Knob knobs[] = {
+      {"tile", &Tuning::tile, {0, 1}},
+      {"cover", &Tuning::cover, {512, 1024, 2048, 4096}},
+      {"pruners", &Tuning::pruners, {256, 512, 1024, 2048}},
+      {"min_bucket", &Tuning::min_bucket, {4, 16, 64, 1u << 30}},
+      {"grid", &Tuning::grid, {256, 512, 1024, 2048}},
+  };
+  std::vector<float> best_ms(G, 0.0f);
+  int rc = SKY_OK;
+  for (const Knob& k : knobs) {
+    std::vector<uint32_t> best(G);
+    std::vector<float> bms(G, 1e30f);
+    for (uint32_t v : k.values) {
+      for (BatchGroup* g : B.groups) c->tune[g->D].*(k.field) = v;
+      for (size_t gi = 0; gi < G && rc == SKY_OK; ++gi) {
+        BatchGroup& g = *B.groups[gi];
+        if (hipEventRecord(ev[2 * gi], c->stream) != hipSuccess) rc = SKY_ERR_DEVICE;
+        if (rc == SKY_OK) SKY_DISPATCH_D(g.D, { rc = batch_group<D>(c, g, rows_dev); });
+        if (rc == SKY_OK && hipEventRecord(ev[2 * gi + 1], c->stream) != hipSuccess)
+          rc = SKY_ERR_DEVICE;
+      }
+      if (rc == SKY_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = SKY_ERR_DEVICE;
+      if (rc != SKY_OK) break;
+      for (size_t gi = 0; gi < G; ++gi) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[2 * gi], ev[2 * gi + 1]) != hipSuccess) rc = SKY_ERR_DEVICE;
+        if (ms < bms[gi]) {
+          bms[gi] = ms;
+          best[gi] = v;
+        }
+      }
+    }
+    if (rc != SKY_OK) break;
+    for (size_t gi = 0; gi < G; ++gi) {
+      c->tune[B.groups[gi]->D].*(k.field) = best[gi];
+      best_ms[gi] = bms[gi];
+    }
+  }
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  c->tune_ver++;
+  c->has_batch = false;  // the tuning passes overwrote the results
+  if (rc != SKY_OK) return fail(c, rc, "autotuning pass failed");
+  if (report && report_cap) {
+    std::string out = "{";
+    for (size_t gi = 0; gi < G; ++gi) {
+      const int D = B.groups[gi]->D;
+      const Tuning& t = c->tune[D];
+      char buf[256];
+      std::snprintf(buf, sizeof(buf),
+                    "%s\"%d\": {\"instances\": %u, \"tile\": %u, \"cover\": %u, "
+                    "\"pruners\": %u, \"min_bucket\": %u, \"grid\": %u, \"ms\": %.3f}",
+                    gi ? ", " : "", D, B.groups[gi]->T, t.tile, t.cover, t.pruners,
+                    t.min_bucket, t.grid, best_ms[gi]);
+      out += buf;
+    }
+    out += "}";
+    std::snprintf(report, report_cap, "%s", out.c_str());
+  }
+  return SKY_OK;
 }
 
 int sky_ctx_release(sky_ctx* c) {
@@ -1276,7 +1423,8 @@ int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n
     rows_dev = P<float>(c->rows);
   }
   BatchState& B = c->batch;
-  if (!batch_same(B, rows_dev, n_inst, n, d, max_d)) TRY(batch_setup(c, rows_dev, n_inst, n, d, max_d));
+  if (!batch_same(B, rows_dev, n_inst, n, d, max_d) || B.sig_tune != c->tune_ver)
+    TRY(batch_setup(c, rows_dev, n_inst, n, d, max_d));
   if (use_graph) {
     if (!B.exec) {
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

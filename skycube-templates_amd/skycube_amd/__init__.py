@@ -26,6 +26,11 @@ _LIB_PATH = os.path.join(_PKG_ROOT, "libskycube_amd.so")
 
 SKY_OK = 0
 SKY_OPT_MEMBER_INDEX = 1
+SKY_OPT_COVER = 2
+SKY_OPT_PRUNERS = 3
+SKY_OPT_MIN_BUCKET = 4
+SKY_OPT_MEMBER_GRID = 5
+SKY_OPT_TILE = 6
 _ERRORS = {
     -1: "SKY_ERR_ARG", -2: "SKY_ERR_NAN", -3: "SKY_ERR_OOM_BUDGET",
     -4: "SKY_ERR_DEVICE", -5: "SKY_ERR_CAPACITY", -6: "SKY_ERR_STATE",
@@ -37,7 +42,7 @@ EXPORTS = (
     "sky_ctx_bytes_in_use", "sky_mdmc", "sky_mdmc_device", "sky_mdmc_fetch",
     "sky_mdmc_device_results", "sky_skyalign", "sky_sdsc", "sky_ctx_set_timing",
     "sky_ctx_stage_times", "sky_mdmc_hashcube", "sky_batch_mdmc", "sky_batch_fetch",
-    "sky_batch_fetch_into",
+    "sky_batch_fetch_into", "sky_ctx_get_option", "sky_batch_autotune",
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
 )
@@ -103,6 +108,10 @@ def load():
     L.sky_ctx_stage_times.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int]
     L.sky_ctx_release.argtypes = [ctypes.c_void_p]
     L.sky_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
+    L.sky_ctx_get_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.sky_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.c_char_p,
+                                     ctypes.c_size_t]
     L.sky_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
     L.sky_server_destroy.argtypes = [ctypes.c_void_p]
@@ -232,6 +241,32 @@ class Context:
     def set_option(self, option: int, value: int) -> None:
         """sky_ctx_set_option; SKY_OPT_MEMBER_INDEX = 1."""
         self._check(self._L.sky_ctx_set_option(self._h, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        self._check(self._L.sky_ctx_get_option(self._h, option, ctypes.byref(v)))
+        return v.value
+
+    def batch_autotune_raw(self, rows_ptr: int, on_device: bool, n, d, max_d=None) -> dict:
+        """sky_batch_autotune over a batch given by a pointer to its rows;
+        returns the chosen settings per dimensionality."""
+        import json as _json
+        n = np.ascontiguousarray(n, dtype=np.uint32)
+        d = np.ascontiguousarray(d, dtype=np.uint32)
+        m = None if max_d is None else np.ascontiguousarray(max_d, dtype=np.uint32)
+        buf = ctypes.create_string_buffer(8192)
+        self._check(self._L.sky_batch_autotune(self._h, ctypes.c_void_p(rows_ptr), int(on_device),
+                                               n.size, _ptr(n, _u32p), _ptr(d, _u32p),
+                                               None if m is None else _ptr(m, _u32p), buf,
+                                               len(buf)))
+        return _json.loads(buf.value.decode())
+
+    def batch_autotune(self, instances, max_d=None) -> dict:
+        mats = [self._rows(x) for x in instances]
+        flat = np.ascontiguousarray(np.concatenate([m.ravel() for m in mats]) if mats
+                                    else np.zeros(1, np.float32))
+        return self.batch_autotune_raw(flat.ctypes.data, False, [m.shape[0] for m in mats],
+                                       [m.shape[1] for m in mats], max_d)
 
     def release(self) -> None:
         """Frees the device buffers the context holds."""

@@ -53,3 +53,46 @@ def test_batch_edge_cases(ctx):
     res = ctx.batch_mdmc([one, dup, one.copy()])
     assert res[0][0].tolist() == [0] and res[2][0].tolist() == [0]
     assert res[1][0].size == 50
+
+
+def test_batch_options_do_not_change_results(ctx):
+    """Extreme settings of every tuning option give the same bitmaps."""
+    import skycube_amd
+    mats = _instances(20, 13, n_hi=2500)
+    ref = ctx.batch_mdmc(mats)
+    settings = [(skycube_amd.SKY_OPT_COVER, 64), (skycube_amd.SKY_OPT_PRUNERS, 64),
+                (skycube_amd.SKY_OPT_MIN_BUCKET, 1), (skycube_amd.SKY_OPT_MEMBER_GRID, 64),
+                (skycube_amd.SKY_OPT_TILE, 1)]
+    with skycube_amd.Context(0) as c:
+        for opt, val in settings:
+            c.set_option(opt, val)
+            assert c.get_option(opt) == val
+        for graph in (False, True):
+            got = c.batch_mdmc(mats, use_graph=graph)
+            for k, ((e1, d1), (e2, d2)) in enumerate(zip(ref, got)):
+                assert np.array_equal(e1, e2) and np.array_equal(d1, d2), k
+        # the single path under the same settings
+        for k in (0, 7, 13):
+            e, dm = c.mdmc(mats[k])
+            assert np.array_equal(e, ref[k][0]) and np.array_equal(dm, ref[k][1]), k
+        with pytest.raises(skycube_amd.SkyError):
+            c.set_option(skycube_amd.SKY_OPT_COVER, 1)
+        with pytest.raises(skycube_amd.SkyError):
+            c.set_option(99, 1)
+
+
+def test_batch_autotune(ctx):
+    import skycube_amd
+    mats = _instances(48, 14, n_hi=3000)
+    ref = ctx.batch_mdmc(mats)
+    with skycube_amd.Context(0) as c:
+        chosen = c.batch_autotune(mats)
+        print(f"autotune: {chosen}", flush=True)
+        dims = sorted({m.shape[1] for m in mats})
+        assert sorted(int(k) for k in chosen) == dims
+        for d, t in chosen.items():
+            assert c.get_option(skycube_amd.SKY_OPT_COVER | (int(d) << 8)) == t["cover"]
+            assert t["tile"] in (0, 1) and t["grid"] in (256, 512, 1024, 2048)
+        got = c.batch_mdmc(mats, use_graph=True)
+        for k, ((e1, d1), (e2, d2)) in enumerate(zip(ref, got)):
+            assert np.array_equal(e1, e2) and np.array_equal(d1, d2), k
