@@ -69,11 +69,13 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
 /* Tuning options of a context (results never depend on them).  No
  * reference counterpart (the reference's kernels have fixed launch shapes).
- *   SKY_OPT_MEMBER_INDEX  how the membership kernel visits candidates: -1
- *                         (default) through median-mask buckets when they
- *                         hold SKY_OPT_MIN_BUCKET or more candidates on
- *                         average, else in score order; 0 always in score
- *                         order; 1 always through the buckets
+ *   SKY_OPT_MEMBER_INDEX  how the membership kernel visits candidates: 0
+ *                         in score order; 1 through median-mask buckets; 2
+ *                         in groups of 64 of the code order, skipped whole
+ *                         by their column-minimum codes; -1 (default): 2 in
+ *                         the single-instance path, and in the batched path
+ *                         1 when the buckets hold SKY_OPT_MIN_BUCKET or
+ *                         more candidates on average, else 0
  *   SKY_OPT_COVER         candidates tested exactly before the coverage
  *                         pruning starts (default 2048; 64 .. 65536)
  *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (1024; 64 .. 8192)

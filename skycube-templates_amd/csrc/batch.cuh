@@ -451,9 +451,18 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
       nb = gfirst[t + 1] - g0;
     }
     const bool use_idx = gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
+    CandIndex ix;
+    if (use_idx) {
+      ix.bmk = bmk + g0;
+      ix.bstart = bstart + g0;
+      ix.bcnt = bcnt + g0;
+      ix.nbk = nb;
+      ix.bidx = blocal;
+      ix.bcode = bcode;
+    }
     member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, m, (int)maxd[t],
-                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover, use_idx,
-                          bmk + g0, bstart + g0, bcnt + g0, nb, blocal, bcode);
+                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover,
+                          use_idx ? kOrderBucket : kOrderScan, ix);
   }
 }
 

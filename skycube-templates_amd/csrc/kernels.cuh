@@ -646,6 +646,28 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
   __syncthreads();
 }
 
+// How k_member visits the candidates after the cover set (order):
+//   kOrderScan    all of them, in score order
+//   kOrderBucket  through median-mask buckets: bucket g holds mask bmk[g] at
+//                 positions bstart[g] .. bstart[g] + bcnt[g] - 1 of the
+//                 bucket order (codes bcode, candidate ids bidx)
+//   kOrderGroup   in groups of 64 consecutive candidates of the code order
+//                 (sorted by the bit-plane code key): gcode / gidx are the
+//                 codes and candidate ids in that order, gsum[g] the
+//                 per-column minimum code of group g as bit planes
+constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2;
+struct CandIndex {
+  const uint32_t* bmk = nullptr;
+  const uint32_t* bstart = nullptr;
+  const uint32_t* bcnt = nullptr;
+  uint32_t nbk = 0;
+  const uint32_t* bidx = nullptr;
+  const uint2* bcode = nullptr;
+  const uint2* gsum = nullptr;
+  const uint2* gcode = nullptr;
+  const uint32_t* gidx = nullptr;
+};
+
 // Per-point cuboid membership, P extended-skyline points per workgroup (the
 // contract of produce_bitmap_iterative_multi_partition_templated_async,
 // src/utilities/kernels.cuh:415-908, without its approximations).
@@ -683,23 +705,16 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 //
 // member_body processes the points base .. base+P-1 of one instance with NT
 // threads; the single-instance kernel and the batched persistent kernel both
-// call it.  With use_idx (uniform) the candidates after the cover set are
-// visited through the median-mask buckets (bucket g: mask bmk[g], positions
-// bstart[g] .. bstart[g] + bcnt[g] - 1 of the bucket order, whose codes are
-// bcode and candidate ids bidx), otherwise all in score order.  Candidate
-// ids are queued as id << 3 | point, so m < 2^29.
+// call it.  order (uniform) and ix say how the candidates after the cover
+// set are visited (CandIndex).  Candidate ids are queued as id << 3 |
+// point, so m < 2^29.
 template <int D, int P, int NT>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
                                             int max_d, const uint32_t* __restrict__ minkey,
                                             uint32_t* __restrict__ dom, uint32_t base,
-                                            uint32_t cover, bool use_idx,
-                                            const uint32_t* __restrict__ bmk,
-                                            const uint32_t* __restrict__ bstart,
-                                            const uint32_t* __restrict__ bcnt, uint32_t nbk,
-                                            const uint32_t* __restrict__ bidx,
-                                            const uint2* __restrict__ bcode) {
+                                            uint32_t cover, int order, const CandIndex& ix) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
   constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
@@ -810,7 +825,49 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
 
-  if (use_idx) {
+  if (order == kOrderGroup) {
+    // Groups of 64 candidates of the code order.  For q in group g, every
+    // code of q is >= the group's column minimum, so the columns where
+    // gsum[g] <= code(p) contain the code bound of every member: if that
+    // set is in A0 the whole group is skipped for p.  Lane l tests group
+    // g0 + l; then the wave code-tests each live group's 64 members at once.
+    const uint32_t ng = (m + 63) / 64;
+    for (uint32_t g0 = (uint32_t)(tid >> 6) * 64; g0 < ng; g0 += (uint32_t)NT) {
+      const uint32_t g = g0 + (uint32_t)lane;
+      uint32_t al = 0;
+      if (g < ng) {
+        const uint2 gs = ix.gsum[g];
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+          const uint32_t ub = planes_le(gs, pc[pp]) & FULL;
+          al |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << pp;
+        }
+      }
+      uint64_t live = __ballot(al != 0u);
+      while (live) {  // uniform
+        const int L = __ffsll((long long)live) - 1;
+        live &= live - 1ull;
+        const uint32_t gal = (uint32_t)__shfl((int)al, L, 64);
+        const uint32_t j = (g0 + (uint32_t)L) * 64u + (uint32_t)lane;
+        uint32_t hit = 0;
+        if (j < m) {
+          const uint2 qc = ix.gcode[j];
+#pragma unroll
+          for (int pp = 0; pp < P; ++pp) {
+            const uint32_t ub = planes_le(qc, pc[pp]) & FULL;
+            const uint32_t w = A0[pp][ub >> 5];
+            hit |= ((~(w >> (ub & 31)) & (gal >> pp)) & 1u) << pp;
+          }
+        }
+        if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+        const uint32_t b = hit ? ix.gidx[j] : 0u;
+        if (b < kc) hit = 0;  // the cover candidates were marked in pass 1a
+#pragma unroll 1
+        for (int pp = 0; pp < P; ++pp) append(hit, pp, pp, b);
+        if (nq >= 64) drain();
+      }
+    }
+  } else if (order == kOrderBucket) {
     // Candidates grouped by median mask (plane 3 of the codes): bucket g
     // holds the candidates of mask bmk[g] at positions bstart[g] ..
     // bstart[g] + bcnt[g] - 1 of the bucket order.  For q in bucket g, L(q, p) lies
@@ -826,19 +883,19 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     uint32_t mp[P];
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) mp[pp] = pc[pp].x >> 16;
-    for (uint32_t g0 = (uint32_t)(tid >> 6) * 64; g0 < nbk; g0 += (uint32_t)NT) {
+    for (uint32_t g0 = (uint32_t)(tid >> 6) * 64; g0 < ix.nbk; g0 += (uint32_t)NT) {
       const uint32_t g = g0 + (uint32_t)lane;
       uint32_t lo = 0, cnt = 0, al = 0;
-      if (g < nbk) {
-        const uint32_t mg = bmk[g];
+      if (g < ix.nbk) {
+        const uint32_t mg = ix.bmk[g];
 #pragma unroll
         for (int pp = 0; pp < P; ++pp) {
           const uint32_t ub = FULL & ~(mg & ~mp[pp]);
           al |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << pp;
         }
         if (al) {
-          lo = bstart[g];
-          cnt = bcnt[g];
+          lo = ix.bstart[g];
+          cnt = ix.bcnt[g];
         }
       }
       uint32_t inc = cnt;
@@ -866,7 +923,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
             if (sEx[l + step] <= e) l += step;
           j = sLo[l] + (e - sEx[l]);
           const uint32_t la = sAl[l];
-          const uint2 qc = bcode[j];
+          const uint2 qc = ix.bcode[j];
 #pragma unroll
           for (int pp = 0; pp < P; ++pp) {
             const uint32_t ub = planes_le(qc, pc[pp]) & FULL;
@@ -876,7 +933,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         }
         if (__ballot(hit != 0u) == 0ull) continue;  // uniform
         // the cover candidates were marked in pass 1a already
-        const uint32_t b = hit ? bidx[j] : 0u;
+        const uint32_t b = hit ? ix.bidx[j] : 0u;
         if (b < kc) hit = 0;
 #pragma unroll 1
         for (int pp = 0; pp < P; ++pp) append(hit, pp, pp, b);
@@ -1055,22 +1112,62 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
 
-template <int D, int P, int NT, bool IDX>
+template <int D, int P, int NT, int ORDER>
 __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                const uint2* __restrict__ codes,
                                                const uint32_t* __restrict__ pos, uint32_t m,
                                                int max_d, const uint32_t* __restrict__ minkey,
                                                uint32_t* __restrict__ dom, uint32_t cover,
-                                               const uint32_t* __restrict__ bmk,
-                                               const uint32_t* __restrict__ bstart,
-                                               const uint32_t* __restrict__ bcnt, uint32_t nbk,
-                                               const uint32_t* __restrict__ bidx,
-                                               const uint2* __restrict__ bcode) {
-  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover, IDX,
-                        bmk, bstart, bcnt, nbk, bidx, bcode);
+                                               CandIndex ix) {
+  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover, ORDER,
+                        ix);
 }
 
 // ---------------------------------------------------------------------------
+// Code order of the candidates (for kOrderGroup).
+
+// 64-bit key of the bit-plane codes, planes 3 to 0 from the top (D bits
+// each), and the candidate's position.
+template <int D>
+__global__ void k_plane_keys(const uint2* __restrict__ codes, uint32_t m,
+                             uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint2 c = codes[j];
+  key[j] = ((uint64_t)(c.x >> 16) << (3 * D)) | ((uint64_t)(c.y >> 16) << (2 * D)) |
+           ((uint64_t)(c.x & 0xFFFFu) << D) | (uint64_t)(c.y & 0xFFFFu);
+  val[j] = j;
+}
+
+// Codes in code order, and each group's per-column minimum code as bit
+// planes (one thread per group of 64).
+template <int D>
+__global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ gidx,
+                            uint32_t m, uint2* __restrict__ gcode, uint2* __restrict__ gsum) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (m + 63) / 64) return;
+  uint32_t mn[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) mn[k] = 15;
+  const uint32_t end = min(m, g * 64 + 64);
+  for (uint32_t j = g * 64; j < end; ++j) {
+    const uint2 c = codes[gidx[j]];
+    gcode[j] = c;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const uint32_t v = ((c.x >> (16 + k)) & 1u) << 3 | ((c.y >> (16 + k)) & 1u) << 2 |
+                         ((c.x >> k) & 1u) << 1 | ((c.y >> k) & 1u);
+      mn[k] = min(mn[k], v);
+    }
+  }
+  uint32_t pl[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pl[b] |= ((mn[k] >> b) & 1u) << k;
+  gsum[g] = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+}
+
 // HashCube export: (word index, word value) -> point ids.
 
 // Words of point i that the HashCube keeps (not all ones), plus the extra

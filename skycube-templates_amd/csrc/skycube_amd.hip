@@ -80,7 +80,7 @@ struct sky_ctx {
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
       masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
-      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff, bcnt;
+      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff, bcnt, gsum;
   // the last batch
   BatchState batch;
   // membership candidate order: -1 by bucket size, 0 scan, 1 bucket index
@@ -403,6 +403,36 @@ int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
   return SKY_OK;
 }
 
+// Code order of the ne extended-skyline points for kOrderGroup: bidx =
+// positions in order of the bit-plane code key, bcode = their codes, gsum =
+// per-column minimum code of every group of 64.
+template <int D>
+int member_groups(sky_ctx* c, uint32_t ne) {
+  const uint32_t ng = (ne + 63) / 64;
+  TRY(ensure(c, c->hc_key_a, sizeof(uint64_t) * ne));
+  TRY(ensure(c, c->hc_key_b, sizeof(uint64_t) * ne));
+  TRY(ensure(c, c->biota, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->bidx, sizeof(uint32_t) * ne));
+  TRY(ensure(c, c->bcode, sizeof(uint2) * ne));
+  TRY(ensure(c, c->gsum, sizeof(uint2) * ng));
+  hipLaunchKernelGGL(sky::k_plane_keys<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                     P<uint2>(c->codes), ne, P<uint64_t>(c->hc_key_a), P<uint32_t>(c->biota));
+  LAUNCHCHK();
+  size_t bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P<uint64_t>(c->hc_key_a),
+                                            P<uint64_t>(c->hc_key_b), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), ne, 0, 4 * D, c->stream));
+  TRY(ensure(c, c->cub_tmp, bytes));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, bytes, P<uint64_t>(c->hc_key_a),
+                                            P<uint64_t>(c->hc_key_b), P<uint32_t>(c->biota),
+                                            P<uint32_t>(c->bidx), ne, 0, 4 * D, c->stream));
+  hipLaunchKernelGGL(sky::k_group_sum<D>, dim3(grid_for(ng)), dim3(kBlock), 0, c->stream,
+                     P<uint2>(c->codes), P<uint32_t>(c->bidx), ne, P<uint2>(c->bcode),
+                     P<uint2>(c->gsum));
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
 int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
              uint32_t* n_ext) {
   c->has_mdmc = false;
@@ -470,24 +500,39 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       LAUNCHCHK();
       TRY(mark(c, SKY_STAGE_MEMBER));
       uint32_t nbk = 0;
-      TRY(member_index(c, d, ne, &nbk));
+      if (c->opt_member_index == sky::kOrderBucket) TRY(member_index(c, d, ne, &nbk));
       constexpr int PP = sky::member_points<D>();
       constexpr int NT = sky::member_threads<D>();
       const Tuning& tu = c->tune[d];
-      const bool idx = c->opt_member_index < 0 ? ne >= (uint64_t)nbk * tu.min_bucket
-                                                : c->opt_member_index != 0;
-      if (idx) {
-        hipLaunchKernelGGL((sky::k_member<D, PP, NT, true>), dim3((ne + PP - 1) / PP), dim3(NT),
-                           0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
-                           P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           tu.cover, P<uint32_t>(c->bkey), P<uint32_t>(c->boff), P<uint32_t>(c->bcnt),
-                           nbk, P<uint32_t>(c->bidx), P<uint2>(c->bcode));
-      } else {
-        hipLaunchKernelGGL((sky::k_member<D, PP, NT, false>), dim3((ne + PP - 1) / PP), dim3(NT),
-                           0, c->stream, P<float>(c->erows), P<uint2>(c->codes),
-                           P<uint32_t>(c->pos), ne, (int)max_d, minkey, P<uint32_t>(c->dom),
-                           tu.cover, nullptr, nullptr, nullptr, 0u, nullptr, nullptr);
+      int order = c->opt_member_index;
+      if (order < 0) order = sky::kOrderGroup;
+      sky::CandIndex ix;
+      if (order == sky::kOrderBucket) {
+        ix.bmk = P<uint32_t>(c->bkey);
+        ix.bstart = P<uint32_t>(c->boff);
+        ix.bcnt = P<uint32_t>(c->bcnt);
+        ix.nbk = nbk;
+        ix.bidx = P<uint32_t>(c->bidx);
+        ix.bcode = P<uint2>(c->bcode);
+      } else if (order == sky::kOrderGroup) {
+        TRY(member_groups<D>(c, ne));
+        ix.gsum = P<uint2>(c->gsum);
+        ix.gcode = P<uint2>(c->bcode);
+        ix.gidx = P<uint32_t>(c->bidx);
       }
+      const dim3 grid((ne + PP - 1) / PP), blk(NT);
+      if (order == sky::kOrderGroup)
+        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderGroup>), grid, blk, 0, c->stream,
+                           P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                           (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
+      else if (order == sky::kOrderBucket)
+        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderBucket>), grid, blk, 0,
+                           c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
+                           ne, (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
+      else
+        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderScan>), grid, blk, 0, c->stream,
+                           P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                           (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
       LAUNCHCHK();
     });
   } else {
@@ -1149,7 +1194,7 @@ void free_all(sky_ctx* c) {
                     &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
                     &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
                     &c->hc_goff, &c->bmask,  &c->biota,  &c->bidx,   &c->bkey,   &c->bcode,
-                    &c->boff,    &c->bcnt};
+                    &c->boff,    &c->bcnt,   &c->gsum};
   for (DevBuf* b : bufs) free_buf(c, *b);
   free_batch_buffers(c);
   c->has_mdmc = c->has_hc = c->has_batch = false;
@@ -1175,7 +1220,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
   const int opt = option & 0xFF, dd = option >> 8;
   if (dd < 0 || dd > 16) return fail(c, SKY_ERR_ARG, "option %d: dimensionality %d", option, dd);
   if (opt == SKY_OPT_MEMBER_INDEX) {
-    if (value < -1 || value > 1 || dd) return fail(c, SKY_ERR_ARG, "bad member index option");
+    if (value < -1 || value > 2 || dd) return fail(c, SKY_ERR_ARG, "bad member index option");
     c->opt_member_index = (int)value;
     return SKY_OK;
   }
