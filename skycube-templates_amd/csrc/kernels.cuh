@@ -731,13 +731,18 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
+  __shared__ uint32_t s_pt[P];  // the points: base + pp of the score order, or of the code order
   __shared__ int s_cnt;
   uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
   constexpr int XCAP = QCAP * 2 < kXCap ? QCAP * 2 : kXCap;
   const int tid = threadIdx.x;
 
   if (tid < P) {
-    const uint32_t a = min(base + tid, m - 1u);
+    // with code-order groups, the workgroup's points are neighbours in the
+    // code order, so that the groups they need coincide
+    const uint32_t bt = min(base + tid, m - 1u);
+    const uint32_t a = order == kOrderGroup ? ix.gidx[bt] : bt;
+    s_pt[tid] = a;
     uint32_t z = 0, g = 0;
     int nz = 0;
     for (int k = 0; k < D; ++k) {
@@ -823,48 +828,75 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   };
   uint2 pc[P];
 #pragma unroll
-  for (int pp = 0; pp < P; ++pp) pc[pp] = codes[min(base + pp, m - 1u)];
+  for (int pp = 0; pp < P; ++pp) pc[pp] = codes[s_pt[pp]];
 
   if (order == kOrderGroup) {
     // Groups of 64 candidates of the code order.  For q in group g, every
     // code of q is >= the group's column minimum, so the columns where
     // gsum[g] <= code(p) contain the code bound of every member: if that
-    // set is in A0 the whole group is skipped for p.  Lane l tests group
-    // g0 + l; then the wave code-tests each live group's 64 members at once.
+    // set is in A0 the whole group is skipped for p.  The waves split the
+    // groups; lane l tests group g0 + l for the workgroup's points (which
+    // are neighbours in the code order, so they mostly need the same
+    // groups), then the wave code-tests the live groups' members, four
+    // groups of 64 at a time.
+    constexpr int NW = NT / 64;
+    constexpr int WPP = NW;
+    const int wv = tid >> 6;
+    const uint32_t mine = (1u << P) - 1u;
     const uint32_t ng = (m + 63) / 64;
-    for (uint32_t g0 = (uint32_t)(tid >> 6) * 64; g0 < ng; g0 += (uint32_t)NT) {
+    for (uint32_t g0 = (uint32_t)(wv % WPP) * 64; g0 < ng; g0 += (uint32_t)WPP * 64) {
       const uint32_t g = g0 + (uint32_t)lane;
       uint32_t al = 0;
       if (g < ng) {
         const uint2 gs = ix.gsum[g];
 #pragma unroll
         for (int pp = 0; pp < P; ++pp) {
+          if (!((mine >> pp) & 1u)) continue;  // uniform
           const uint32_t ub = planes_le(gs, pc[pp]) & FULL;
           al |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << pp;
         }
       }
       uint64_t live = __ballot(al != 0u);
-      while (live) {  // uniform
-        const int L = __ffsll((long long)live) - 1;
-        live &= live - 1ull;
-        const uint32_t gal = (uint32_t)__shfl((int)al, L, 64);
-        const uint32_t j = (g0 + (uint32_t)L) * 64u + (uint32_t)lane;
-        uint32_t hit = 0;
-        if (j < m) {
-          const uint2 qc = ix.gcode[j];
+      while (live) {  // uniform; up to kLive live groups per round, loads in flight together
+        constexpr int kLive = 4;
+        int Ls[kLive];
+#pragma unroll
+        for (int r = 0; r < kLive; ++r) {
+          Ls[r] = live ? __ffsll((long long)live) - 1 : -1;
+          live &= live - 1ull;
+        }
+        uint2 qc[kLive];
+        uint32_t gal[kLive], jj[kLive];
+#pragma unroll
+        for (int r = 0; r < kLive; ++r) {
+          const int L = Ls[r] < 0 ? 0 : Ls[r];
+          gal[r] = Ls[r] < 0 ? 0u : (uint32_t)__shfl((int)al, L, 64);
+          jj[r] = (g0 + (uint32_t)L) * 64u + (uint32_t)lane;
+          qc[r] = ix.gcode[min(jj[r], m - 1u)];
+        }
+        uint32_t hit = 0;  // bit r * P + pp
+#pragma unroll
+        for (int r = 0; r < kLive; ++r) {
 #pragma unroll
           for (int pp = 0; pp < P; ++pp) {
-            const uint32_t ub = planes_le(qc, pc[pp]) & FULL;
+            if (!((gal[r] >> pp) & 1u)) continue;  // uniform
+            const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
             const uint32_t w = A0[pp][ub >> 5];
-            hit |= ((~(w >> (ub & 31)) & (gal >> pp)) & 1u) << pp;
+            hit |= ((~(w >> (ub & 31)) & (uint32_t)(jj[r] < m)) & 1u) << (r * P + pp);
           }
         }
         if (__ballot(hit != 0u) == 0ull) continue;  // uniform
-        const uint32_t b = hit ? ix.gidx[j] : 0u;
-        if (b < kc) hit = 0;  // the cover candidates were marked in pass 1a
 #pragma unroll 1
-        for (int pp = 0; pp < P; ++pp) append(hit, pp, pp, b);
-        if (nq >= 64) drain();
+        for (int r = 0; r < kLive; ++r) {
+          uint32_t h = (hit >> (r * P)) & ((1u << P) - 1u);
+          if (__ballot(h != 0u) == 0ull) continue;  // uniform
+          const uint32_t b = h ? ix.gidx[jj[r]] : 0u;
+          if (b < kc) h = 0;  // the cover candidates were marked in pass 1a
+#pragma unroll 1
+          for (int pp = 0; pp < P; ++pp)
+            if ((mine >> pp) & 1u) append(h, pp, pp, b);
+          if (nq >= 64) drain();
+        }
       }
     }
   } else if (order == kOrderBucket) {
@@ -947,7 +979,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     // code-tests kPer candidates against the P points per round without
     // branches, the next round's codes already in flight.
     constexpr int kPer = 4;
-    static_assert(kPer * P <= 32, "survivor mask is one word");
+    static_assert(kPer * P <= 32, "survivor mask is one word");  // also 4 live groups x P
     constexpr uint32_t kStep = (uint32_t)NT * kPer;
     uint2 qn[kPer];
 #pragma unroll
@@ -1106,7 +1138,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) {
     if (base + pp >= m) break;
-    const uint32_t row = pos[base + pp];
+    const uint32_t row = pos[s_pt[pp]];
     for (int k = tid; k < W; k += NT) dom[(size_t)row * W + k] = A1[pp][k];
   }
   __syncthreads();  // the LDS may be reused by the caller's next block
