@@ -354,6 +354,13 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": B},
     }
+    # the same skycube through host buffers (sky_mdmc: rows up, ids and
+    # bitmaps down over PCIe into pageable memory), for reference only
+    t0 = time.perf_counter()
+    h_ext, h_dom = ctx.mdmc(x, max_d)
+    out["host_buffers_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    out["host_buffers_bytes"] = int(x.nbytes + h_ext.nbytes + h_dom.nbytes)
+    del h_ext, h_dom
     if args.verify:
         import hashlib
         gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))

@@ -63,13 +63,17 @@ def test_fixture_score_ties():
     norms are where STSC's score early exit could differ from the exact
     sets.  The NBA file, the samples and two of the full workloads hold
     none; the full anticorrelated workload holds 379 (sums of 12 values on
-    the plane sum(x) = 6 round to equal FP32 values)."""
+    the plane sum(x) = 6 round to equal FP32 values), and in every one of
+    them p has another dominator in s with a strictly smaller norm, which
+    STSC meets first (orc_score_tie_unresolved = 0), so STSC computes the
+    definition on every fixture input."""
     assert META["nba"]["score_tie_triples"] == 0
     for m in META["samples"].values():
         assert m["score_tie_triples"] == 0
     full = META.get("full", {})
     for name, m in full.items():
         assert m["score_tie_triples"] == (379 if name == "anticorrelated_plane" else 0), name
+        assert m["score_tie_unresolved"] == 0, name
 
 
 @pytest.mark.gpu
