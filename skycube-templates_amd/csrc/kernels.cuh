@@ -647,75 +647,25 @@ struct CandIndex {
   const uint32_t* gidx = nullptr;
 };
 
-// Word-level closures of NA (1 or 3) P x W bitmaps at once.  Level l
-// (word-index bit l) ORs word k | 2^l into word k (down) or, for the levels
-// set in up[pp] of array 2, word k into word k | 2^l (up).  Only levels
-// below lim[pp] (per point) are applied.  Per-level transforms commute, so
-// levels can go in any order and several at once: levels 0-2 as sets of 8
-// contiguous words (two 16-byte loads per thread), levels 3-5 one by one
-// (pairs of runs of >= 8 words: no bank conflicts), levels from 6 on as
-// sets of 8 words 2^l apart (consecutive threads, consecutive words).  One
-// workgroup barrier per step; callers sync before; ends with a barrier.
-template <int P, int W, int NT, int NA>
-__device__ __forceinline__ void close_step(uint32_t (*A)[W], uint32_t (*B)[W], uint32_t (*C)[W],
-                                           const uint32_t* up, const int* lim, int c0, int nb) {
-  const int s0 = 1 << c0;
-  const int nsets = W >> nb;
-  for (int idx = threadIdx.x; idx < NA * P * nsets; idx += NT) {
-    const int q = idx % nsets;
-    const int pa = idx / nsets;
-    const int pp = pa % P, a = pa / P;
-    uint32_t(*X)[W] = a == 0 ? A : (a == 1 ? B : C);
-    const int base = ((q >> c0) << (c0 + nb)) | (q & (s0 - 1));
-    const uint32_t u = a == 2 ? up[pp] >> c0 : 0u;
-    const int nl = lim ? min(nb, max(0, lim[pp] - c0)) : nb;
-    uint32_t v[8];
-    if (s0 == 1 && nb == 3) {
-      const uint4 lo = *reinterpret_cast<const uint4*>(&X[pp][base]);
-      const uint4 hi = *reinterpret_cast<const uint4*>(&X[pp][base + 4]);
-      v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
-      v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
-    } else {
+// Down-closure of the P bitmaps A (W words each) in place; caller syncs
+// before.  Ends with a barrier.
+template <int D, int P, int W, int NT>
+__device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
+  for (int k = threadIdx.x; k < W; k += NT) {
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (t < (1 << nb)) v[t] = X[pp][base + t * s0];
-    }
+    for (int pp = 0; pp < P; ++pp) A[pp][k] = low_closure(A[pp][k], D);
+  }
+#pragma unroll 1
+  for (int j = 5; j < D; ++j) {
+    __syncthreads();
+    const int sh = j - 5;
+    for (int t = threadIdx.x; t < W / 2; t += NT) {
+      const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      if (l >= nl) break;
-      const int bit = 1 << l;
-      if ((u >> l) & 1u) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < (1 << nb) && (t & bit)) v[t] |= v[t ^ bit];
-      } else {
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < (1 << nb) && !(t & bit)) v[t] |= v[t | bit];
-      }
-    }
-    if (s0 == 1 && nb == 3) {
-      *reinterpret_cast<uint4*>(&X[pp][base]) = make_uint4(v[0], v[1], v[2], v[3]);
-      *reinterpret_cast<uint4*>(&X[pp][base + 4]) = make_uint4(v[4], v[5], v[6], v[7]);
-    } else {
-#pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (t < (1 << nb)) X[pp][base + t * s0] = v[t];
+      for (int pp = 0; pp < P; ++pp) A[pp][lo] |= A[pp][lo | (1 << sh)];
     }
   }
   __syncthreads();
-}
-
-template <int P, int W, int L, int NT, int NA>
-__device__ __forceinline__ void close_sets(uint32_t (*A)[W], uint32_t (*B)[W], uint32_t (*C)[W],
-                                           const uint32_t* up, const int* lim) {
-  if (L <= 0) return;
-  close_step<P, W, NT, NA>(A, B, C, up, lim, 0, L < 3 ? L : 3);
-#pragma unroll
-  for (int c0 = 3; c0 < 6 && c0 < L; ++c0) close_step<P, W, NT, NA>(A, B, C, up, lim, c0, 1);
-#pragma unroll
-  for (int c0 = 6; c0 < L; c0 += 3)
-    close_step<P, W, NT, NA>(A, B, C, up, lim, c0, L - c0 < 3 ? L - c0 : 3);
 }
 
 // Per-point cuboid membership, P extended-skyline points per workgroup (the
@@ -906,14 +856,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       if (nq >= 64) drain();
     }
   };
-  auto close_a0 = [&]() {
-    for (int k = tid; k < W; k += NT) {
-#pragma unroll
-      for (int pp = 0; pp < P; ++pp) A0[pp][k] = low_closure(A0[pp][k], D);
-    }
-    __syncthreads();
-    close_sets<P, W, (D > 5 ? D - 5 : 0), NT, 1>(A0, nullptr, nullptr, nullptr, nullptr);
-  };
+  auto close_a0 = [&]() { close_down<D, P, W, NT>(A0); };
 
   // pass 1a: the cover.  Its first quarter is tested pair by pair; A0 is
   // closed; the rest of the cover is code-tested against that A0 (in
@@ -1092,9 +1035,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   for (int k = tid; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
-      const uint32_t a0 = A0[pp][k];
-      A0[pp][k] = low_closure(a0, D);
-      A1[pp][k] = low_closure(A1[pp][k] | a0, D);
+      A1[pp][k] |= A0[pp][k];
       uint32_t t = TE[pp][k];
       const uint32_t z = s_zm[pp];
       if (D > 0) t |= (z & 1u) ? (t & 0x55555555u) << 1 : (t & 0xAAAAAAAAu) >> 1;
@@ -1105,17 +1046,35 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       TE[pp][k] = t;
     }
   }
-  // word levels: column j >= 5 is word-index bit j - 5 (TE goes up on Z);
-  // U: level j < |Z| of the compressed Z lattice
-  __shared__ uint32_t s_up[P];
-  __shared__ int s_lim[P];
-  if (tid < P) {
-    s_up[tid] = s_zm[tid] >> 5;
-    s_lim[tid] = __popc(s_zm[tid]);
-  }
   __syncthreads();
-  close_sets<P, W, (D > 5 ? D - 5 : 0), NT, 3>(A0, A1, TE, s_up, nullptr);
-  close_sets<P, NU, ZB, NT, 1>(U, nullptr, nullptr, nullptr, s_lim);
+  close_down<D, P, W, NT>(A0);
+  close_down<D, P, W, NT>(A1);
+#pragma unroll 1
+  for (int j = 5; j < D; ++j) {
+    const int sh = j - 5;
+    for (int t = tid; t < W / 2; t += NT) {
+      const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
+      const int hi = lo | (1 << sh);
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp) {
+        if (s_zm[pp] >> j & 1u)
+          TE[pp][hi] |= TE[pp][lo];
+        else
+          TE[pp][lo] |= TE[pp][hi];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll 1
+  for (int j = 0; j < ZB; ++j) {
+    for (int i = tid; i < NU / 2; i += NT) {
+      const int lo = ((i >> j) << (j + 1)) | (i & ((1 << j) - 1));
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp)
+        if (j < __popc(s_zm[pp])) U[pp][lo] |= U[pp][lo | (1 << j)];
+    }
+    __syncthreads();
+  }
 
   // per point: final words; X bits are parked in TE for the second pass
 #pragma unroll 1
