@@ -84,6 +84,9 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *                         kernel (1024; 64 .. 8192)
  *   SKY_OPT_TILE          points per membership workgroup in the batched
  *                         path: 0 the default, 1 half of it
+ *   SKY_OPT_BATCH_ORDER   candidate order of the batched membership: 2
+ *                         (default) code-order groups, 0 scan or buckets
+ *                         as SKY_OPT_MIN_BUCKET decides
  * option | (d << 8) sets the value for dimensionality d only (d = 1..16);
  * a plain option sets it for every d.  sky_ctx_get_option reads one back
  * (for d = 1 when no d is given). */
@@ -93,6 +96,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 #define SKY_OPT_MIN_BUCKET 4
 #define SKY_OPT_MEMBER_GRID 5
 #define SKY_OPT_TILE 6
+#define SKY_OPT_BATCH_ORDER 7
 int sky_ctx_set_option(sky_ctx *ctx, int option, int64_t value);
 int sky_ctx_get_option(sky_ctx *ctx, int option, int64_t *value);
 
@@ -198,10 +202,11 @@ int sky_batch_fetch(sky_ctx *ctx, uint64_t *ext_off, uint32_t *ext_pids,
                     uint64_t cap_words);
 
 /* Autotuner of the batched path: runs the batch (as sky_batch_mdmc would)
- * once per candidate value of each setting in turn -- tile (SKY_OPT_TILE),
- * cover depth, prefilter size, bucket threshold, persistent grid -- timing
+ * once per candidate value of each setting in turn -- candidate order,
+ * tile (SKY_OPT_TILE), cover depth, prefilter size, bucket threshold,
+ * persistent grid -- timing
  * every dimensionality group with HIP events, and keeps for each group's d
- * the fastest value (coordinate descent, about 19 passes; tune on a sample
+ * the fastest value (coordinate descent, about 21 passes; tune on a sample
  * of a large batch).  The chosen settings stay on the context; report (if
  * not NULL) receives them as JSON text.  The last batch's results are not
  * kept.  No reference counterpart. */

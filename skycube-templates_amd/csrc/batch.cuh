@@ -354,6 +354,47 @@ __global__ void k_blocal(const uint32_t* __restrict__ ids, uint32_t cnt,
   out[j] = i - off[seg_of(off, T, i)];
 }
 
+// Code order per instance (for kOrderGroup): 64-bit plane keys of the
+// entries j < *count (all ones past it), sorted per instance segment.
+template <int D>
+__global__ void k_bplane_keys(const uint2* __restrict__ codes, const uint32_t* __restrict__ count,
+                              uint32_t N, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  val[j] = j;
+  if (j >= *count) {
+    key[j] = ~0ull;
+    return;
+  }
+  const uint2 c = codes[j];
+  key[j] = ((uint64_t)(c.x >> 16) << (3 * D)) | ((uint64_t)(c.y >> 16) << (2 * D)) |
+           ((uint64_t)(c.x & 0xFFFFu) << D) | (uint64_t)(c.y & 0xFFFFu);
+}
+
+// Groups of 64 per instance: cnt[t] = ceil(m_t / 64), cnt[T] = 0 (for the
+// exclusive scan into group offsets).
+__global__ void k_bgroup_counts(const uint32_t* __restrict__ eoff, uint32_t T,
+                                uint32_t* __restrict__ cnt) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > T) return;
+  cnt[t] = t < T ? (eoff[t + 1] - eoff[t] + 63) / 64 : 0u;
+}
+
+// Group g (of all instances, goff[T] in total) of instance t: codes in code
+// order, candidate ids local to the instance, per-column minimum codes.
+template <int D>
+__global__ void k_bgroup_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ sval,
+                             const uint32_t* __restrict__ eoff, uint32_t T,
+                             const uint32_t* __restrict__ goff, uint2* __restrict__ gcode,
+                             uint32_t* __restrict__ glocal, uint2* __restrict__ gsum) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= goff[T]) return;
+  const uint32_t t = seg_of(goff, T, g);
+  const uint32_t e0 = eoff[t], e1 = eoff[t + 1];
+  const uint32_t j0 = e0 + (g - goff[t]) * 64;
+  group_sum<D>(codes, sval, j0, min(e1, j0 + 64), e0, gcode, glocal, gsum + g);
+}
+
 // flag_pt[spid[j]] = 1 and flag_s1[j] = 1 for survivors nothing strictly
 // dominates; flag_s1 is cleared up to N.
 __global__ void k_bext_flags(const uint8_t* __restrict__ status, const uint32_t* __restrict__ spid,
@@ -414,7 +455,9 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 }
 
 // Membership, persistent: each workgroup takes (instance, block) tasks from
-// a device counter until the group's tasks (toff[T]) are done.  With gfirst,
+// a device counter until the group's tasks (toff[T]) are done.  With goff,
+// every instance visits its candidates in code-order groups (instance t's
+// groups from goff[t] on; gcode / glocal in the group code order); else with gfirst,
 // instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
 // (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
 // hold min_bucket candidates or more on average; cover as in member_body.
@@ -429,6 +472,10 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
                                                 uint32_t* __restrict__ dom,
                                                 uint32_t* __restrict__ work, uint32_t cover,
                                                 uint32_t min_bucket,
+                                                const uint32_t* __restrict__ goff,
+                                                const uint2* __restrict__ gsum,
+                                                const uint2* __restrict__ gcode,
+                                                const uint32_t* __restrict__ glocal,
                                                 const uint32_t* __restrict__ gfirst,
                                                 const uint32_t* __restrict__ bmk,
                                                 const uint32_t* __restrict__ bstart,
@@ -452,7 +499,13 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
     }
     const bool use_idx = gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
     CandIndex ix;
-    if (use_idx) {
+    int order = use_idx ? kOrderBucket : kOrderScan;
+    if (goff) {
+      order = kOrderGroup;
+      ix.gsum = gsum + goff[t];
+      ix.gcode = gcode + e0;
+      ix.gidx = glocal + e0;
+    } else if (use_idx) {
       ix.bmk = bmk + g0;
       ix.bstart = bstart + g0;
       ix.bcnt = bcnt + g0;
@@ -461,8 +514,7 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
       ix.bcode = bcode;
     }
     member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, m, (int)maxd[t],
-                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover,
-                          use_idx ? kOrderBucket : kOrderScan, ix);
+                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover, order, ix);
   }
 }
 

@@ -1187,20 +1187,22 @@ __global__ void k_plane_keys(const uint2* __restrict__ codes, uint32_t m,
   val[j] = j;
 }
 
-// Codes in code order, and each group's per-column minimum code as bit
-// planes (one thread per group of 64).
+// Group of the code order at positions j0 .. j1-1: the codes in that order
+// (gcode[j] = codes[sidx[j]]), the candidate ids minus bias (gidx, if
+// given) and the per-column minimum code as bit planes (*gsum).
 template <int D>
-__global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ gidx,
-                            uint32_t m, uint2* __restrict__ gcode, uint2* __restrict__ gsum) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (m + 63) / 64) return;
+__device__ __forceinline__ void group_sum(const uint2* __restrict__ codes,
+                                          const uint32_t* __restrict__ sidx, uint32_t j0,
+                                          uint32_t j1, uint32_t bias, uint2* __restrict__ gcode,
+                                          uint32_t* __restrict__ gidx, uint2* __restrict__ gsum) {
   uint32_t mn[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) mn[k] = 15;
-  const uint32_t end = min(m, g * 64 + 64);
-  for (uint32_t j = g * 64; j < end; ++j) {
-    const uint2 c = codes[gidx[j]];
+  for (uint32_t j = j0; j < j1; ++j) {
+    const uint32_t i = sidx[j];
+    const uint2 c = codes[i];
     gcode[j] = c;
+    if (gidx) gidx[j] = i - bias;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const uint32_t v = ((c.x >> (16 + k)) & 1u) << 3 | ((c.y >> (16 + k)) & 1u) << 2 |
@@ -1213,7 +1215,16 @@ __global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __r
   for (int k = 0; k < D; ++k)
 #pragma unroll
     for (int b = 0; b < 4; ++b) pl[b] |= ((mn[k] >> b) & 1u) << k;
-  gsum[g] = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+  *gsum = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+}
+
+// Codes in code order and each group's summary (one thread per group of 64).
+template <int D>
+__global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ gidx,
+                            uint32_t m, uint2* __restrict__ gcode, uint2* __restrict__ gsum) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (m + 63) / 64) return;
+  group_sum<D>(codes, gidx, g * 64, min(m, g * 64 + 64), 0u, gcode, nullptr, gsum + g);
 }
 
 // HashCube export: (word index, word value) -> point ids.

@@ -56,6 +56,8 @@ struct BatchState {
   // mask index: sort keys / values, bucket order, bucket list, bucket table
   DevBuf ikey, ikey2, ival, isort, ibkey, ibcode, iblocal, ihead, ibmk, ibstart, ibcnt, ibrun,
       igfirst, iboff;
+  // code-order groups: per-instance group counts / offsets, summaries
+  DevBuf gcnt, goff, gsum;
   size_t tmp_bytes = 0;
 };
 
@@ -66,6 +68,7 @@ struct Tuning {
   uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
   uint32_t grid = 1024;                   // workgroups of the persistent kernel
   uint32_t tile = 0;                      // 0: member_points, 1: half of it
+  uint32_t order = 2;                     // batched membership: 2 code-order groups, 0 scan / buckets
 };
 
 }  // namespace
@@ -725,7 +728,8 @@ void free_batch_buffers(sky_ctx* c) {
                   &B.status, &B.flag_pt, &B.flag_s1, &B.eidx, &B.erows, &B.epid,  &B.pos,
                   &B.codes, &B.rank,   &B.piv,    &B.minkey, &B.seg_a, &B.seg_b, &B.tmp,
                   &B.ikey,  &B.ikey2,  &B.ival,   &B.isort,  &B.ibkey, &B.ibcode, &B.iblocal,
-                  &B.ihead, &B.ibmk,   &B.ibstart, &B.ibcnt, &B.ibrun, &B.igfirst, &B.iboff};
+                  &B.ihead, &B.ibmk,   &B.ibstart, &B.ibcnt, &B.ibrun, &B.igfirst, &B.iboff,
+                  &B.gcnt,  &B.goff,   &B.gsum};
   for (DevBuf* b : sb) free_buf(c, *b);
   B.tmp_bytes = 0;
 }
@@ -763,6 +767,10 @@ int batch_tmp_bytes(sky_ctx* c, uint32_t N, uint32_t T, size_t* out) {
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                             (uint32_t*)nullptr, (uint32_t*)nullptr, N, 0, 32,
                                             c->stream));
+  mx = std::max(mx, b);
+  HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(
+      nullptr, b, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, N,
+      T, (const uint32_t*)nullptr, (const uint32_t*)nullptr, 0, 64, c->stream));
   mx = std::max(mx, b);
   HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b, it, (uint8_t*)nullptr, (uint32_t*)nullptr,
                                         (uint32_t*)nullptr, N, c->stream));
@@ -869,6 +877,9 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   TRY(ensure(c, B.ibcode, sizeof(uint2) * N));
   TRY(ensure(c, B.ihead, N));
   TRY(ensure(c, B.igfirst, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.gcnt, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.goff, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.gsum, sizeof(uint2) * (N / 64 + T + 1)));
   uint64_t tab = 1;
   for (const BatchGroup* g : B.groups)
     if (batch_self_index(g->D, g->T)) tab = std::max<uint64_t>(tab, ((uint64_t)g->T << g->D) + 1);
@@ -952,8 +963,11 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
                      P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                      P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
                      P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.min_bucket,
-                     P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk), P<uint32_t>(B.ibstart),
-                     P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal), P<uint2>(B.ibcode));
+                     tu.order == 2 ? P<uint32_t>(B.goff) : nullptr, P<uint2>(B.gsum),
+                     P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
+                     tu.order == 2 ? nullptr : P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
+                     P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
+                     P<uint2>(B.ibcode));
   LAUNCHCHK();
   return SKY_OK;
 }
@@ -1058,24 +1072,46 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.erows),
                      eoff, T, cnt + 3, P<uint32_t>(B.piv), P<uint2>(B.codes));
   LAUNCHCHK();
-  // bucket list of the extended skyline per instance (count in cnt[5])
-  TRY(batch_mask_sort<D>(c, eoff, T, N, cnt + 3));
-  hipLaunchKernelGGL(sky::k_bidx_heads, dim3(gN), dim3(kBlock), 0, c->stream, P<uint32_t>(B.ikey2),
-                     cnt + 3, N, P<uint8_t>(B.ihead));
-  LAUNCHCHK();
-  TRY(select_dev(c, P<uint8_t>(B.ihead), N, P<uint32_t>(B.ival), cnt + 5));
-  hipLaunchKernelGGL(sky::k_bidx_list<D>, dim3(gN), dim3(kBlock), 0, c->stream,
-                     P<uint32_t>(B.ikey2), P<uint32_t>(B.ival), cnt + 5, cnt + 3, N,
-                     P<uint32_t>(B.ibmk), P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt),
-                     P<uint32_t>(B.ibrun));
-  LAUNCHCHK();
-  hipLaunchKernelGGL(sky::k_bidx_first, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream,
-                     P<uint32_t>(B.ibrun), cnt + 5, T, P<uint32_t>(B.igfirst));
-  LAUNCHCHK();
-  hipLaunchKernelGGL(sky::k_bidx_gather, dim3(gN), dim3(kBlock), 0, c->stream,
-                     P<uint32_t>(B.isort), nullptr, P<uint2>(B.codes), cnt + 3, eoff, T, nullptr,
-                     P<uint2>(B.ibcode), P<uint32_t>(B.iblocal));
-  LAUNCHCHK();
+  if (tu.order == 2) {
+    // code order per instance, groups of 64 with their summaries
+    hipLaunchKernelGGL(sky::k_bplane_keys<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint2>(B.codes), cnt + 3, N, P<uint64_t>(B.key_a), P<uint32_t>(B.ival));
+    LAUNCHCHK();
+    size_t bb = B.tmp_bytes;
+    HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        B.tmp.p, bb, P<uint64_t>(B.key_a), P<uint64_t>(B.key_b), P<uint32_t>(B.ival),
+        P<uint32_t>(B.isort), N, T, eoff, eoff + 1, 0, 4 * D, c->stream));
+    hipLaunchKernelGGL(sky::k_bgroup_counts, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream,
+                       eoff, T, P<uint32_t>(B.gcnt));
+    LAUNCHCHK();
+    bb = B.tmp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, bb, P<uint32_t>(B.gcnt), P<uint32_t>(B.goff),
+                                            T + 1, c->stream));
+    hipLaunchKernelGGL(sky::k_bgroup_sum<D>, dim3(grid_for(N / 64 + T + 1)), dim3(kBlock), 0,
+                       c->stream, P<uint2>(B.codes), P<uint32_t>(B.isort), eoff, T,
+                       P<uint32_t>(B.goff), P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
+                       P<uint2>(B.gsum));
+    LAUNCHCHK();
+  } else {
+    // bucket list of the extended skyline per instance (count in cnt[5])
+    TRY(batch_mask_sort<D>(c, eoff, T, N, cnt + 3));
+    hipLaunchKernelGGL(sky::k_bidx_heads, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.ikey2), cnt + 3, N, P<uint8_t>(B.ihead));
+    LAUNCHCHK();
+    TRY(select_dev(c, P<uint8_t>(B.ihead), N, P<uint32_t>(B.ival), cnt + 5));
+    hipLaunchKernelGGL(sky::k_bidx_list<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.ikey2), P<uint32_t>(B.ival), cnt + 5, cnt + 3, N,
+                       P<uint32_t>(B.ibmk), P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt),
+                       P<uint32_t>(B.ibrun));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bidx_first, dim3(grid_for(T + 1)), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.ibrun), cnt + 5, T, P<uint32_t>(B.igfirst));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bidx_gather, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.isort), nullptr, P<uint2>(B.codes), cnt + 3, eoff, T,
+                       nullptr, P<uint2>(B.ibcode), P<uint32_t>(B.iblocal));
+    LAUNCHCHK();
+  }
   if (tu.tile == 0)
     TRY((batch_member<D, sky::member_points<D>()>(c, g, eoff, tu)));
   else
@@ -1235,6 +1271,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
       {SKY_OPT_MIN_BUCKET, 1, 1 << 30, &Tuning::min_bucket},
       {SKY_OPT_MEMBER_GRID, 64, 1 << 13, &Tuning::grid},
       {SKY_OPT_TILE, 0, 1, &Tuning::tile},
+      {SKY_OPT_BATCH_ORDER, 0, 2, &Tuning::order},
   };
   for (const Range& r : ranges) {
     if (r.opt != opt) continue;
@@ -1260,6 +1297,7 @@ int sky_ctx_get_option(sky_ctx* c, int option, int64_t* value) {
     case SKY_OPT_MIN_BUCKET: *value = t.min_bucket; return SKY_OK;
     case SKY_OPT_MEMBER_GRID: *value = t.grid; return SKY_OK;
     case SKY_OPT_TILE: *value = t.tile; return SKY_OK;
+    case SKY_OPT_BATCH_ORDER: *value = t.order; return SKY_OK;
     default: return fail(c, SKY_ERR_ARG, "unknown option %d", option);
   }
 }
@@ -1283,6 +1321,7 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
     std::vector<uint32_t> values;
   };
   const Knob knobs[] = {
+      {"order", &Tuning::order, {2, 0}},
       {"tile", &Tuning::tile, {0, 1}},
       {"cover", &Tuning::cover, {512, 1024, 2048, 4096}},
       {"pruners", &Tuning::pruners, {256, 512, 1024, 2048}},
@@ -1331,9 +1370,10 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
       const Tuning& t = c->tune[D];
       char buf[256];
       std::snprintf(buf, sizeof(buf),
-                    "%s\"%d\": {\"instances\": %u, \"tile\": %u, \"cover\": %u, "
-                    "\"pruners\": %u, \"min_bucket\": %u, \"grid\": %u, \"ms\": %.3f}",
-                    gi ? ", " : "", D, B.groups[gi]->T, t.tile, t.cover, t.pruners,
+                    "%s\"%d\": {\"instances\": %u, \"order\": %u, \"tile\": %u, "
+                    "\"cover\": %u, \"pruners\": %u, \"min_bucket\": %u, \"grid\": %u, "
+                    "\"ms\": %.3f}",
+                    gi ? ", " : "", D, B.groups[gi]->T, t.order, t.tile, t.cover, t.pruners,
                     t.min_bucket, t.grid, best_ms[gi]);
       out += buf;
     }

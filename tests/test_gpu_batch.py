@@ -63,6 +63,14 @@ def test_batch_options_do_not_change_results(ctx):
     settings = [(skycube_amd.SKY_OPT_COVER, 64), (skycube_amd.SKY_OPT_PRUNERS, 64),
                 (skycube_amd.SKY_OPT_MIN_BUCKET, 1), (skycube_amd.SKY_OPT_MEMBER_GRID, 64),
                 (skycube_amd.SKY_OPT_TILE, 1)]
+    for order in (0, 2):  # buckets (min_bucket 1), code-order groups
+        with skycube_amd.Context(0) as c:
+            c.set_option(skycube_amd.SKY_OPT_BATCH_ORDER, order)
+            for opt, val in settings:
+                c.set_option(opt, val)
+            got = c.batch_mdmc(mats)
+            for k, ((e1, d1), (e2, d2)) in enumerate(zip(ref, got)):
+                assert np.array_equal(e1, e2) and np.array_equal(d1, d2), (order, k)
     with skycube_amd.Context(0) as c:
         for opt, val in settings:
             c.set_option(opt, val)
@@ -93,6 +101,7 @@ def test_batch_autotune(ctx):
         for d, t in chosen.items():
             assert c.get_option(skycube_amd.SKY_OPT_COVER | (int(d) << 8)) == t["cover"]
             assert t["tile"] in (0, 1) and t["grid"] in (256, 512, 1024, 2048)
+            assert t["order"] in (0, 2)
         got = c.batch_mdmc(mats, use_graph=True)
         for k, ((e1, d1), (e2, d2)) in enumerate(zip(ref, got)):
             assert np.array_equal(e1, e2) and np.array_equal(d1, d2), k
