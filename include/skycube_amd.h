@@ -78,6 +78,9 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *                         more candidates on average, else 0
  *   SKY_OPT_COVER         candidates tested exactly before the coverage
  *                         pruning starts (default 2048; 64 .. 65536)
+ *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
+ *                         the rest of the cover is code-tested against the
+ *                         closure of the head (0: a quarter of the cover)
  *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (1024; 64 .. 8192)
  *   SKY_OPT_MIN_BUCKET    bucket-size threshold above (16; 1 .. 2^30)
  *   SKY_OPT_MEMBER_GRID   workgroups of the batched persistent membership
@@ -97,6 +100,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 #define SKY_OPT_MEMBER_GRID 5
 #define SKY_OPT_TILE 6
 #define SKY_OPT_BATCH_ORDER 7
+#define SKY_OPT_COVER_HEAD 8
 int sky_ctx_set_option(sky_ctx *ctx, int option, int64_t value);
 int sky_ctx_get_option(sky_ctx *ctx, int option, int64_t *value);
 

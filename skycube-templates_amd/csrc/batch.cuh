@@ -471,7 +471,7 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
                                                 const uint32_t* __restrict__ minkey,
                                                 uint32_t* __restrict__ dom,
                                                 uint32_t* __restrict__ work, uint32_t cover,
-                                                uint32_t min_bucket,
+                                                uint32_t head, uint32_t min_bucket,
                                                 const uint32_t* __restrict__ goff,
                                                 const uint2* __restrict__ gsum,
                                                 const uint2* __restrict__ gcode,
@@ -499,6 +499,7 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
     }
     const bool use_idx = gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
     CandIndex ix;
+    ix.head = head;
     int order = use_idx ? kOrderBucket : kOrderScan;
     if (goff) {
       order = kOrderGroup;

@@ -17,7 +17,7 @@
 namespace sky {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
-constexpr uint32_t kPruners = 1024;  // default: lowest-score points used as a prefilter
+constexpr uint32_t kPruners = 512;  // default: lowest-score points used as a prefilter
 constexpr int kXCap = 2048;       // tie-touched subspaces resolved per round
 constexpr int kCodePivots = 15;   // 16 quantile buckets per column, 4-bit codes
 
@@ -636,6 +636,7 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
 //                 per-column minimum code of group g as bit planes
 constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2;
 struct CandIndex {
+  uint32_t head = 0;  // cover candidates tested pair by pair (0: a quarter of the cover)
   const uint32_t* bmk = nullptr;
   const uint32_t* bstart = nullptr;
   const uint32_t* bcnt = nullptr;
@@ -858,11 +859,11 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   };
   auto close_a0 = [&]() { close_down<D, P, W, NT>(A0); };
 
-  // pass 1a: the cover.  Its first quarter is tested pair by pair; A0 is
+  // pass 1a: the cover.  Its head (a quarter by default) is tested pair by pair; A0 is
   // closed; the rest of the cover is code-tested against that A0 (in
   // score order, the likely dominators first); A0 is closed again.
   const uint32_t kc = min(m, cover);
-  const uint32_t kc0 = min(kc, max(64u, cover / 4));
+  const uint32_t kc0 = min(kc, ix.head ? ix.head : max(64u, cover / 4));
   for (uint32_t e = tid; e < kc0 * P; e += NT) {
     const uint32_t b = e / P;
     const int pp = (int)(e - b * P);
@@ -1113,11 +1114,35 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   }
   __syncthreads();
 
-  // second pass for X (residual-tie subspaces): uniform per point
+  // second pass for X (residual-tie subspaces): uniform per point.  A
+  // candidate can settle an X subspace s only if s lies in L(q, p), inside
+  // its code bound ub, and if it ties p in a column outside Z, where their
+  // codes are equal too.  So X is closed upwards into A0 (free by now) and a
+  // candidate whose ub is not in that closure, or whose codes equal p's in
+  // no column outside Z, is skipped before its row is read.
 #pragma unroll 1
   for (int pp = 0; pp < P; ++pp) {
     if (!s_any[pp]) continue;
     const uint32_t z = s_zm[pp];
+    const uint2 pcp = codes[s_pt[pp]];
+    for (int k = tid; k < W; k += NT) {
+      uint32_t x = TE[pp][k];
+      if (D > 0) x |= (x & 0x55555555u) << 1;
+      if (D > 1) x |= (x & 0x33333333u) << 2;
+      if (D > 2) x |= (x & 0x0F0F0F0Fu) << 4;
+      if (D > 3) x |= (x & 0x00FF00FFu) << 8;
+      if (D > 4) x |= (x & 0x0000FFFFu) << 16;
+      A0[pp][k] = x;
+    }
+#pragma unroll 1
+    for (int j = 5; j < D; ++j) {
+      __syncthreads();
+      const int sh = j - 5;
+      for (int t = tid; t < W / 2; t += NT) {
+        const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
+        A0[pp][lo | (1 << sh)] |= A0[pp][lo];
+      }
+    }
 #pragma unroll 1
     while (true) {
       __syncthreads();
@@ -1138,6 +1163,11 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       const int cnt = min(s_cnt, XCAP);
       if (cnt == 0) break;
       for (uint32_t b = tid; b < m; b += NT) {
+        const uint2 qc = codes[b];
+        const uint32_t ub = planes_le(qc, pcp) & FULL;
+        if (!((A0[pp][ub >> 5] >> (ub & 31)) & 1u)) continue;
+        const uint32_t ex = ~(qc.x ^ pcp.x), ey = ~(qc.y ^ pcp.y);
+        if (((ex >> 16) & ex & (ey >> 16) & ey & FULL & ~z) == 0) continue;
         uint32_t lt, eq;
         pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
         if (lt == 0 || (eq & ~z) == 0) continue;
@@ -1155,7 +1185,11 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   for (int pp = 0; pp < P; ++pp) {
     if (base + pp >= m) break;
     const uint32_t row = pos[s_pt[pp]];
-    for (int k = tid; k < W; k += NT) dom[(size_t)row * W + k] = A1[pp][k];
+    // streaming stores: the bitmaps are not read again here, and keeping
+    // them out of L2 leaves it to the candidate codes the other workgroups
+    // are scanning
+    for (int k = tid; k < W; k += NT)
+      __builtin_nontemporal_store(A1[pp][k], &dom[(size_t)row * W + k]);
   }
   __syncthreads();  // the LDS may be reused by the caller's next block
 }

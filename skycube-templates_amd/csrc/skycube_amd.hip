@@ -69,6 +69,7 @@ struct Tuning {
   uint32_t grid = 1024;                   // workgroups of the persistent kernel
   uint32_t tile = 0;                      // 0: member_points, 1: half of it
   uint32_t order = 2;                     // batched membership: 2 code-order groups, 0 scan / buckets
+  uint32_t head = 0;                      // cover head tested pair by pair (0: cover / 4)
 };
 
 }  // namespace
@@ -510,6 +511,7 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       int order = c->opt_member_index;
       if (order < 0) order = sky::kOrderGroup;
       sky::CandIndex ix;
+      ix.head = tu.head;
       if (order == sky::kOrderBucket) {
         ix.bmk = P<uint32_t>(c->bkey);
         ix.bstart = P<uint32_t>(c->boff);
@@ -962,7 +964,7 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
   hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
                      P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                      P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                     P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.min_bucket,
+                     P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head, tu.min_bucket,
                      tu.order == 2 ? P<uint32_t>(B.goff) : nullptr, P<uint2>(B.gsum),
                      P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
                      tu.order == 2 ? nullptr : P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
@@ -1272,6 +1274,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
       {SKY_OPT_MEMBER_GRID, 64, 1 << 13, &Tuning::grid},
       {SKY_OPT_TILE, 0, 1, &Tuning::tile},
       {SKY_OPT_BATCH_ORDER, 0, 2, &Tuning::order},
+      {SKY_OPT_COVER_HEAD, 0, 1 << 16, &Tuning::head},
   };
   for (const Range& r : ranges) {
     if (r.opt != opt) continue;
@@ -1298,6 +1301,7 @@ int sky_ctx_get_option(sky_ctx* c, int option, int64_t* value) {
     case SKY_OPT_MEMBER_GRID: *value = t.grid; return SKY_OK;
     case SKY_OPT_TILE: *value = t.tile; return SKY_OK;
     case SKY_OPT_BATCH_ORDER: *value = t.order; return SKY_OK;
+    case SKY_OPT_COVER_HEAD: *value = t.head; return SKY_OK;
     default: return fail(c, SKY_ERR_ARG, "unknown option %d", option);
   }
 }

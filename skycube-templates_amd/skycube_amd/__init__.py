@@ -32,6 +32,7 @@ SKY_OPT_MIN_BUCKET = 4
 SKY_OPT_MEMBER_GRID = 5
 SKY_OPT_TILE = 6
 SKY_OPT_BATCH_ORDER = 7
+SKY_OPT_COVER_HEAD = 8
 _ERRORS = {
     -1: "SKY_ERR_ARG", -2: "SKY_ERR_NAN", -3: "SKY_ERR_OOM_BUDGET",
     -4: "SKY_ERR_DEVICE", -5: "SKY_ERR_CAPACITY", -6: "SKY_ERR_STATE",
