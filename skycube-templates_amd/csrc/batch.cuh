@@ -332,7 +332,8 @@ __global__ __launch_bounds__(kBlock) void k_bself_idx(
     const uint2* __restrict__ codes, const uint32_t* __restrict__ off, uint32_t T,
     const uint32_t* __restrict__ count, const uint32_t* __restrict__ boff,
     const uint32_t* __restrict__ sval, const uint32_t* __restrict__ bkey,
-    const uint2* __restrict__ bcode, uint8_t* __restrict__ status) {
+    const uint2* __restrict__ bcode, const uint32_t* __restrict__ minkey,
+    uint8_t* __restrict__ status) {
   __shared__ uint32_t sLo[kBlock / 64][64];
   __shared__ uint32_t sEx[kBlock / 64][64];
   const int wv = threadIdx.x >> 6;
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_bself_idx(
   if (j >= *count) return;  // whole wave; no block barrier in this kernel
   const uint32_t t = seg_of(off, T, j);
   const uint8_t st = self_idx<D>(rows, keys, codes, boff + ((size_t)t << D), sval, bkey, bcode, j,
-                                 0, sLo[wv], sEx[wv]);
+                                 0, minkey + (size_t)t * 16, sLo[wv], sEx[wv]);
   if ((threadIdx.x & 63) == 0) status[j] = st;
 }
 

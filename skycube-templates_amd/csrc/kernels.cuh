@@ -283,7 +283,9 @@ __global__ void k_bucket_list(const uint32_t* __restrict__ smask, const uint32_t
 // the subsets of mask(i), find their bucket prefixes (key <= key(i)) and
 // scan the concatenation 64 entries at a time: code test, then the exact
 // FP32 test of the few that pass.  The wave stops at a strict dominator.
-// The SkyAlign survivor pass (src/utilities/kernels.cuh:36-411) with its
+// With minkey (the column minima of the candidates) and !need_dom, a point
+// that holds a column minimum is answered at once: nothing is strictly
+// below it there.  The SkyAlign survivor pass (src/utilities/kernels.cuh:36-411) with its
 // partition containment test turned into an index: only candidates that
 // pass the containment test are visited.  sLo / sEx: 64 words each of the
 // wave's LDS.
@@ -295,12 +297,19 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
                                             const uint32_t* __restrict__ bidx,
                                             const uint32_t* __restrict__ bkey,
                                             const uint2* __restrict__ bcode, uint32_t i,
-                                            int need_dom, uint32_t* sLo, uint32_t* sEx) {
+                                            int need_dom, const uint32_t* __restrict__ minkey,
+                                            uint32_t* sLo, uint32_t* sEx) {
   constexpr uint32_t FULL = (1u << D) - 1u;
   const int lane = threadIdx.x & 63;
   float p[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) p[k] = rows[(size_t)i * D + k];
+  if (minkey && !need_dom) {  // uniform: a point holding a column minimum
+    bool at_min = false;       // has no strict dominator
+#pragma unroll
+    for (int k = 0; k < D; ++k) at_min |= order_key(p[k]) == minkey[k];
+    if (at_min) return 0;
+  }
   const uint32_t kp = keys[i];
   const uint2 pc = codes[i];
   const uint32_t M = pc.x >> 16;  // plane 3: the median mask
@@ -381,14 +390,14 @@ __global__ __launch_bounds__(kBlock) void k_self_idx(
     const uint2* __restrict__ codes,
     const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bidx,
     const uint32_t* __restrict__ bkey, const uint2* __restrict__ bcode, uint32_t m, int need_dom,
-    uint8_t* __restrict__ status) {
+    const uint32_t* __restrict__ minkey, uint8_t* __restrict__ status) {
   __shared__ uint32_t sLo[kBlock / 64][64];
   __shared__ uint32_t sEx[kBlock / 64][64];
   const int wv = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * (kBlock / 64) + wv;
   if (i >= m) return;  // whole wave; the kernel has no block barrier
   const uint8_t st = self_idx<D>(rows, keys, codes, boff, bidx, bkey, bcode, i, need_dom,
-                                 sLo[wv], sEx[wv]);
+                                 minkey, sLo[wv], sEx[wv]);
   if ((threadIdx.x & 63) == 0) status[i] = st;
 }
 

@@ -58,6 +58,7 @@ struct BatchState {
       igfirst, iboff;
   // code-order groups: per-instance group counts / offsets, summaries
   DevBuf gcnt, goff, gsum;
+  DevBuf sminkey;  // column minima of the survivors per instance
   size_t tmp_bytes = 0;
 };
 
@@ -347,12 +348,20 @@ int run_split(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, ui
     LAUNCHCHK();
   });
   TRY(self_index(c, ds, m));
+  // column minima of the survivors (a point holding one has no strict dominator)
+  uint32_t* sminkey = P<uint32_t>(c->counter) + 32;
+  if (!need_dom) HIPCHK(hipMemsetAsync(sminkey, 0xFF, 16 * sizeof(uint32_t), c->stream));
   SKY_DISPATCH_D(ds, {
+    if (!need_dom) {
+      hipLaunchKernelGGL(sky::k_colmin<D>, dim3(std::min<unsigned>(grid_for(m), 1024u)),
+                         dim3(kBlock), 0, c->stream, P<float>(c->s1rows), m, sminkey);
+      LAUNCHCHK();
+    }
     hipLaunchKernelGGL(sky::k_self_idx<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
                        P<float>(c->s1rows), P<uint32_t>(c->s1keys),
                        P<uint2>(c->codes), P<uint32_t>(c->boff), P<uint32_t>(c->bidx),
                        P<uint32_t>(c->bkey), P<uint2>(c->bcode), m, need_dom,
-                       P<uint8_t>(c->status));
+                       need_dom ? nullptr : sminkey, P<uint8_t>(c->status));
     LAUNCHCHK();
   });
   *m_out = m;
@@ -731,7 +740,7 @@ void free_batch_buffers(sky_ctx* c) {
                   &B.codes, &B.rank,   &B.piv,    &B.minkey, &B.seg_a, &B.seg_b, &B.tmp,
                   &B.ikey,  &B.ikey2,  &B.ival,   &B.isort,  &B.ibkey, &B.ibcode, &B.iblocal,
                   &B.ihead, &B.ibmk,   &B.ibstart, &B.ibcnt, &B.ibrun, &B.igfirst, &B.iboff,
-                  &B.gcnt,  &B.goff,   &B.gsum};
+                  &B.gcnt,  &B.goff,   &B.gsum,   &B.sminkey};
   for (DevBuf* b : sb) free_buf(c, *b);
   B.tmp_bytes = 0;
 }
@@ -871,6 +880,7 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   TRY(ensure(c, B.rank, sizeof(uint32_t) * N));
   TRY(ensure(c, B.piv, sizeof(uint32_t) * T * 16 * sky::kCodePivots));
   TRY(ensure(c, B.minkey, sizeof(uint32_t) * T * 16));
+  TRY(ensure(c, B.sminkey, sizeof(uint32_t) * T * 16));
   TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
   TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
   DevBuf* ib[] = {&B.ikey, &B.ikey2, &B.ival, &B.isort, &B.ibkey, &B.iblocal, &B.ibmk, &B.ibstart,
@@ -1031,10 +1041,13 @@ int batch_group(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
                        P<uint32_t>(B.isort), P<uint32_t>(B.s1keys), P<uint2>(B.codes), cnt + 1,
                        nullptr, T, P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr);
     LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bcolmin<D>, dim3(T), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), s1off, P<uint32_t>(B.sminkey));
+    LAUNCHCHK();
     hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
                        P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes), s1off, T,
                        cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort), P<uint32_t>(B.ibkey),
-                       P<uint2>(B.ibcode), P<uint8_t>(B.status));
+                       P<uint2>(B.ibcode), P<uint32_t>(B.sminkey), P<uint8_t>(B.status));
     LAUNCHCHK();
   } else {
     hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream,
