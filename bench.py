@@ -350,7 +350,8 @@ def main() -> int:
                    "cuboids": cuboids_in_scope(d, max_d), "n_ext": n_ext,
                    "result": "device-resident membership bitmaps (2^d bits per ext point)"},
         "stages_ms": st,
-        "roofline": {"bound": "hbm", "kernel": f"k_member<{d}>", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": f"k_member<d={d}> (code-order groups)",
+                     "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": B},
     }
