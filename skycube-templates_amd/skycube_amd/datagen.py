@@ -105,6 +105,9 @@ WORKLOADS = {
     "independent_distinct": dict(n=100_000, d=8, max_d=8, seed=1),
     "anticorrelated_plane": dict(n=500_000, d=12, max_d=12, seed=2),
     "integer_ties": dict(n=1_000_000, d=16, max_d=8, seed=3),
+    # n = 1M, d = 16 independent data (distinct values): 635,442 points in
+    # the extended skyline
+    "independent_16d": dict(n=1_000_000, d=16, max_d=8, seed=6),
 }
 
 
@@ -119,6 +122,8 @@ def workload(name: str, n: int | None = None) -> np.ndarray:
         return anticorrelated(size, w["d"], w["seed"])
     if name == "integer_ties":
         return integer_ties(size, w["d"], w["seed"])
+    if name == "independent_16d":
+        return independent(size, w["d"], w["seed"], distinct=True)
     raise KeyError(name)
 
 

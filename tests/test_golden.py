@@ -90,7 +90,7 @@ def test_gpu_samples(ctx, name, wl, n, max_d):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("wl", sorted(datagen.WORKLOADS))
+@pytest.mark.parametrize("wl", sorted(META.get("full", {})))
 def test_gpu_full_workload_digest(ctx, wl):
     """Bit-exact at full size: n_ext and the sha256 of (ext pids, bitmaps)."""
     full = META.get("full", {})

@@ -185,3 +185,25 @@ def test_mdmc_member_candidate_order(d, mode):
         x = rng.random((n, d), dtype=np.float32)
         check_mdmc(c, x, max(1, d - 3), tag=f"order {mode} continuous")
         check_mdmc(c, datagen.anticorrelated(n, d, 50 + d), tag=f"order {mode} anticorrelated")
+
+
+def test_mdmc_independent_16d_sampled(ctx):
+    """n = 1M, d = 16 independent data (635,442 extended-skyline points):
+    too large for a full oracle run, so checked by sampling: sampled points
+    of the extended skyline have no strict dominator and their bitmaps equal
+    the oracle's; sampled points outside it have one."""
+    w = datagen.WORKLOADS["independent_16d"]
+    x = datagen.workload("independent_16d")
+    _say(f"independent_16d n={x.shape[0]} d={x.shape[1]} max_d={w['max_d']}")
+    ext, dom = ctx.mdmc(x, w["max_d"])
+    assert ext.size == 635442
+    rng = np.random.default_rng(7)
+    smp = np.sort(rng.choice(ext.size, size=48, replace=False))
+    odom = skyorc.mdmc_sample(x, ext, smp, w["max_d"])
+    assert np.array_equal(dom[smp], odom)
+    inside = np.zeros(x.shape[0], dtype=bool)
+    inside[ext] = True
+    for i in ext[smp[:16]]:
+        assert not np.all(x < x[i], axis=1).any(), i
+    for i in rng.choice(np.flatnonzero(~inside), size=16, replace=False):
+        assert np.all(x < x[i], axis=1).any(), i
