@@ -76,8 +76,10 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *                         the single-instance path, and in the batched path
  *                         1 when the buckets hold SKY_OPT_MIN_BUCKET or
  *                         more candidates on average, else 0
- *   SKY_OPT_COVER         candidates tested exactly before the coverage
- *                         pruning starts (default 2048; 64 .. 65536)
+ *   SKY_OPT_COVER         lowest-norm candidates covered before the
+ *                         coverage pruning starts (64 .. 65536; 0, the
+ *                         default: about n_ext / 40 as a power of two in
+ *                         [2048, 16384] for one skycube, 2048 in a batch)
  *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
  *                         the rest of the cover is code-tested against the
  *                         closure of the head (0: a quarter of the cover)

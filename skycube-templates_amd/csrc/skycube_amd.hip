@@ -64,7 +64,7 @@ struct BatchState {
 
 // Launch settings of one dimensionality (sky_ctx_set_option, autotuner).
 struct Tuning {
-  uint32_t cover = sky::kCover;           // candidates tested exactly first
+  uint32_t cover = 0;  // candidates covered first; 0: by n_ext (single) / kCover (batched)
   uint32_t pruners = sky::kPruners;       // prefilter points
   uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
   uint32_t grid = 1024;                   // workgroups of the persistent kernel
@@ -519,6 +519,13 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       const Tuning& tu = c->tune[d];
       int order = c->opt_member_index;
       if (order < 0) order = sky::kOrderGroup;
+      // the cover grows with the extended skyline: about n_ext / 40, a power
+      // of two in [2048, 16384] (measured best on the three large workloads)
+      uint32_t cover = tu.cover;
+      if (!cover) {
+        cover = 2048;
+        while (cover < 16384 && (uint64_t)cover * 2 * 40 <= ne) cover *= 2;
+      }
       sky::CandIndex ix;
       ix.head = tu.head;
       if (order == sky::kOrderBucket) {
@@ -538,15 +545,15 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       if (order == sky::kOrderGroup)
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderGroup>), grid, blk, 0, c->stream,
                            P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                           (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
+                           (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
       else if (order == sky::kOrderBucket)
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderBucket>), grid, blk, 0,
                            c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
-                           ne, (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
+                           ne, (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
       else
         hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderScan>), grid, blk, 0, c->stream,
                            P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                           (int)max_d, minkey, P<uint32_t>(c->dom), tu.cover, ix);
+                           (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
       LAUNCHCHK();
     });
   } else {
@@ -974,7 +981,8 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
   hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
                      P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                      P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                     P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head, tu.min_bucket,
+                     P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
+                     tu.min_bucket,
                      tu.order == 2 ? P<uint32_t>(B.goff) : nullptr, P<uint2>(B.gsum),
                      P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
                      tu.order == 2 ? nullptr : P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
@@ -1281,7 +1289,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
     uint32_t Tuning::*field;
   };
   static const Range ranges[] = {
-      {SKY_OPT_COVER, 64, 1 << 16, &Tuning::cover},
+      {SKY_OPT_COVER, 0, 1 << 16, &Tuning::cover},
       {SKY_OPT_PRUNERS, 64, 1 << 13, &Tuning::pruners},
       {SKY_OPT_MIN_BUCKET, 1, 1 << 30, &Tuning::min_bucket},
       {SKY_OPT_MEMBER_GRID, 64, 1 << 13, &Tuning::grid},
@@ -1291,7 +1299,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
   };
   for (const Range& r : ranges) {
     if (r.opt != opt) continue;
-    if (value < r.lo || value > r.hi)
+    if (value < r.lo || value > r.hi || (opt == SKY_OPT_COVER && value > 0 && value < 64))
       return fail(c, SKY_ERR_ARG, "option %d: value %lld outside [%lld, %lld]", opt,
                   (long long)value, (long long)r.lo, (long long)r.hi);
     for (int k = dd ? dd : 1; k <= (dd ? dd : 16); ++k) c->tune[k].*(r.field) = (uint32_t)value;

@@ -85,6 +85,9 @@ def test_batch_options_do_not_change_results(ctx):
             assert np.array_equal(e, ref[k][0]) and np.array_equal(dm, ref[k][1]), k
         with pytest.raises(skycube_amd.SkyError):
             c.set_option(skycube_amd.SKY_OPT_COVER, 1)
+        c.set_option(skycube_amd.SKY_OPT_COVER, 0)  # back to the size rule
+        e, dm = c.mdmc(mats[3])
+        assert np.array_equal(e, ref[3][0]) and np.array_equal(dm, ref[3][1])
         with pytest.raises(skycube_amd.SkyError):
             c.set_option(99, 1)
 
