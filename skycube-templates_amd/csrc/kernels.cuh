@@ -525,10 +525,12 @@ constexpr uint32_t kCover = 2048;  // default: lowest-score candidates tested fi
 // costs one bound test and a share of a 64-lane round), else scans them.
 constexpr uint32_t kMinBucket = 16;
 
-// Points per workgroup of k_member: as many as keep its LDS near 64 KB.
+// Points per workgroup of k_member: as many as keep its LDS near 64 KB,
+// and no more than 4 from d = 11 on (measured: the points of a workgroup
+// share its candidate groups; more points make more groups live).
 template <int D>
 constexpr int member_points() {
-  return D >= 15 ? 2 : (D >= 13 ? 4 : 8);
+  return D >= 15 ? 2 : (D >= 11 ? 4 : 8);
 }
 
 // The other tile width the batch autotuner may pick: half as many points.
