@@ -32,6 +32,9 @@ import numpy as np  # noqa: E402
 
 METRIC = "ms/skycube (n=1M,d=16) vs CPU; batched inst/s vs plain MDMC; p99 latency"
 HBM_PEAK_GBS = 8000.0
+# wave64 VALU instructions per second the chip can issue: 256 CUs x 4 SIMDs,
+# one instruction per SIMD every 4 clocks, at 2.4 GHz
+VALU_PEAK_IPS = 256 * 4 * 2.4e9 / 4
 
 
 def cuboids_in_scope(d: int, max_d: int) -> int:
@@ -341,6 +344,17 @@ def main() -> int:
     tpath = os.path.join(ROOT, "profiles", "k_member_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get(args.workload)
+    # the kernel is bound by instruction issue, not bytes: its VALU
+    # instruction count per launch (rocprofv3, profiles/) over the live time
+    valu = None
+    cpath = os.path.join(ROOT, "profiles", "k_member_counters.json")
+    if os.path.exists(cpath):
+        cnt = json.load(open(cpath)).get(args.workload)
+        if cnt:
+            ips = cnt["valu_insts"] / (member_ms * 1e-3)
+            valu = {"bound": "valu", "achieved": round(ips / 1e9, 1), "peak": VALU_PEAK_IPS / 1e9,
+                    "unit": "G wave64 VALU instructions/s", "frac": round(ips / VALU_PEAK_IPS, 3),
+                    "insts_per_launch": cnt["valu_insts"], "source": "profiles/k_member_counters.json"}
     out = {
         "metric": METRIC, "value": round(ms_step, 3), "unit": "ms/skycube", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -354,6 +368,7 @@ def main() -> int:
                      "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": B},
+        "valu_roofline": valu,
     }
     # the same skycube through host buffers (sky_mdmc: rows up, ids and
     # bitmaps down over PCIe into pageable memory), for reference only
