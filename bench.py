@@ -277,6 +277,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="integer_ties")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-hashcube", action="store_true",
+                    help="single mode: skip the HashCube export timing")
     ap.add_argument("--verify", action="store_true", help="sha256 of the result vs the golden digest")
     ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
                     help="single: one skycube per step (the headline); batch: the batched "
@@ -369,6 +371,9 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": B},
         "valu_roofline": valu,
+        # implementation-independent: ext-skyline pairs per second (pruning
+        # can take it past any peak; for comparing implementations only)
+        "ext_pair_rate_per_s": float("%.4g" % (n_ext * n_ext / (ms_step * 1e-3))),
     }
     # the same skycube through host buffers (sky_mdmc: rows up, ids and
     # bitmaps down over PCIe into pageable memory), for reference only
@@ -377,6 +382,23 @@ def main() -> int:
     out["host_buffers_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
     out["host_buffers_bytes"] = int(x.nbytes + h_ext.nbytes + h_dom.nbytes)
     del h_ext, h_dom
+    if not args.no_hashcube:
+        # the reference's timed window (src/main.h:324-336: rows already in
+        # host memory, result materialised as a HashCube): rows up, the
+        # skycube, the HashCube grouped on the device and copied to host
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xd2 = torch.from_numpy(x).to("cuda")
+        torch.cuda.synchronize()
+        ctx.mdmc_device(xd2.data_ptr(), n, d, max_d)
+        t1 = time.perf_counter()
+        gw, gv, go, ids = ctx.mdmc_hashcube()
+        t2 = time.perf_counter()
+        out["reference_window_ms"] = round((t2 - t0) * 1e3, 1)
+        out["hashcube"] = {"ms": round((t2 - t1) * 1e3, 1), "groups": int(gw.size),
+                           "entries": int(ids.size),
+                           "what": "(word, value) -> ids grouped on the device, copied to host"}
+        del xd2, gw, gv, go, ids
     if args.verify:
         import hashlib
         gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))

@@ -1043,11 +1043,14 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   drain();
   __syncthreads();
 
-  // closures: A0 and A1 = A0|A1 down; TE down on R, up on Z; U superset-OR
+  // closures: A0 and A1 = A0|A1 down; TE down on R, up on Z; U superset-OR.
+  // All four in one sweep of levels, one barrier per level.
   for (int k = tid; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
-      A1[pp][k] |= A0[pp][k];
+      const uint32_t a0 = A0[pp][k];
+      A0[pp][k] = low_closure(a0, D);
+      A1[pp][k] = low_closure(A1[pp][k] | a0, D);
       uint32_t t = TE[pp][k];
       const uint32_t z = s_zm[pp];
       if (D > 0) t |= (z & 1u) ? (t & 0x55555555u) << 1 : (t & 0xAAAAAAAAu) >> 1;
@@ -1058,35 +1061,36 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       TE[pp][k] = t;
     }
   }
-  __syncthreads();
-  close_down<D, P, W, NT>(A0);
-  close_down<D, P, W, NT>(A1);
+  constexpr int kLevels = (D > 5 ? D - 5 : 0) > ZB ? (D > 5 ? D - 5 : 0) : ZB;
 #pragma unroll 1
-  for (int j = 5; j < D; ++j) {
-    const int sh = j - 5;
-    for (int t = tid; t < W / 2; t += NT) {
-      const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
-      const int hi = lo | (1 << sh);
+  for (int lv = 0; lv < kLevels; ++lv) {
+    __syncthreads();
+    if (lv + 5 < D) {  // uniform: word level j = lv + 5
+      const int sh = lv;
+      for (int t = tid; t < W / 2; t += NT) {
+        const int lo = ((t >> sh) << (sh + 1)) | (t & ((1 << sh) - 1));
+        const int hi = lo | (1 << sh);
 #pragma unroll
-      for (int pp = 0; pp < P; ++pp) {
-        if (s_zm[pp] >> j & 1u)
-          TE[pp][hi] |= TE[pp][lo];
-        else
-          TE[pp][lo] |= TE[pp][hi];
+        for (int pp = 0; pp < P; ++pp) {
+          A0[pp][lo] |= A0[pp][hi];
+          A1[pp][lo] |= A1[pp][hi];
+          if (s_zm[pp] >> (lv + 5) & 1u)
+            TE[pp][hi] |= TE[pp][lo];
+          else
+            TE[pp][lo] |= TE[pp][hi];
+        }
       }
     }
-    __syncthreads();
-  }
-#pragma unroll 1
-  for (int j = 0; j < ZB; ++j) {
-    for (int i = tid; i < NU / 2; i += NT) {
-      const int lo = ((i >> j) << (j + 1)) | (i & ((1 << j) - 1));
+    if (lv < ZB) {  // uniform: U level lv
+      for (int i = tid; i < NU / 2; i += NT) {
+        const int lo = ((i >> lv) << (lv + 1)) | (i & ((1 << lv) - 1));
 #pragma unroll
-      for (int pp = 0; pp < P; ++pp)
-        if (j < __popc(s_zm[pp])) U[pp][lo] |= U[pp][lo | (1 << j)];
+        for (int pp = 0; pp < P; ++pp)
+          if (lv < __popc(s_zm[pp])) U[pp][lo] |= U[pp][lo | (1 << lv)];
+      }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
   // per point: final words; X bits are parked in TE for the second pass
 #pragma unroll 1
