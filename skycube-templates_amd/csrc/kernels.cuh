@@ -519,6 +519,14 @@ template <int D>
 constexpr int zbits() {
   return D < 8 ? D : (D >= 15 ? 10 : 8);
 }
+// At d = 16 the single-instance kernel runs with a table of 2^8 words (and
+// 64-entry queues), so that three workgroups fit a CU's LDS; the few points
+// with more minimum columns are deferred to a second launch with the full
+// table.  (At d = 15 registers, not LDS, set the occupancy.)
+template <int D>
+constexpr int narrow_zbits() {
+  return D == 16 ? 8 : zbits<D>();
+}
 constexpr uint32_t kCover = 2048;  // default: lowest-score candidates tested first
 // Default: k_member visits the candidates through the median-mask buckets
 // when they hold at least this many candidates on average (each bucket
@@ -657,6 +665,13 @@ struct CandIndex {
   const uint2* gsum = nullptr;
   const uint2* gcode = nullptr;
   const uint32_t* gidx = nullptr;
+  // the points: pts[0 .. npts-1] if set (else positions 0 .. m-1, through
+  // gidx in the group order)
+  const uint32_t* pts = nullptr;
+  uint32_t npts = 0;
+  // if set, a point with more minimum columns than the kernel's boundary
+  // table holds is not computed but appended to defer[1 ..] (count defer[0])
+  uint32_t* defer = nullptr;
 };
 
 // Down-closure of the P bitmaps A (W words each) in place; caller syncs
@@ -720,7 +735,7 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // call it.  order (uniform) and ix say how the candidates after the cover
 // set are visited (CandIndex).  Candidate ids are queued as id << 3 |
 // point, so m < 2^29.
-template <int D, int P, int NT>
+template <int D, int P, int NT, int ZB = zbits<D>()>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
@@ -728,9 +743,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             uint32_t* __restrict__ dom, uint32_t base,
                                             uint32_t cover, int order, const CandIndex& ix) {
   constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
-  constexpr int ZB = zbits<D>();
   constexpr int NU = 1 << ZB;
-  constexpr int QW = 128;              // queue entries per wave
+  constexpr int QW = D == 16 ? 64 : 128;  // queue entries per wave
   constexpr int QCAP = (NT / 64) * QW;
   constexpr uint32_t FULL = (1u << D) - 1u;
   __shared__ uint32_t A0[P][W];
@@ -744,6 +758,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
   __shared__ uint32_t s_pt[P];  // the points: base + pp of the score order, or of the code order
+  __shared__ uint32_t s_ok;     // bit pp: point pp exists and is computed here
   __shared__ int s_cnt;
   uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
   constexpr int XCAP = QCAP * 2 < kXCap ? QCAP * 2 : kXCap;
@@ -752,8 +767,10 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   if (tid < P) {
     // with code-order groups, the workgroup's points are neighbours in the
     // code order, so that the groups they need coincide
-    const uint32_t bt = min(base + tid, m - 1u);
-    const uint32_t a = order == kOrderGroup ? ix.gidx[bt] : bt;
+    const uint32_t np = ix.pts ? ix.npts : m;
+    const uint32_t bt = min(base + tid, np - 1u);
+    const uint32_t a = ix.pts ? ix.pts[bt] : (order == kOrderGroup ? ix.gidx[bt] : bt);
+    bool ok = base + tid < np;
     s_pt[tid] = a;
     uint32_t z = 0, g = 0;
     int nz = 0;
@@ -767,6 +784,12 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       }
       g |= (uint32_t)(!at_min) << k;
     }
+    if (ix.defer && ok && __popc(~g & FULL) > ZB) {
+      ix.defer[1 + atomicAdd(ix.defer, 1u)] = a;
+      ok = false;
+    }
+    const uint32_t okm = (uint32_t)__ballot(ok);  // the P lanes are in wave 0
+    if (tid == 0) s_ok = okm;
     s_zm[tid] = z;
     s_rgt[tid] = g;
     s_any[tid] = 0;
@@ -785,6 +808,10 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   }
   if (tid == 0) s_cnt = 0;
   __syncthreads();
+  if (s_ok == 0) {  // uniform: no point of this workgroup is computed here
+    __syncthreads();
+    return;
+  }
 
   // Candidate passes.  Survivors of the code test (rare) are appended to
   // the wave's queue (tested on the spot when it is full), and a wave
@@ -905,7 +932,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     constexpr int NW = NT / 64;
     constexpr int WPP = NW;
     const int wv = tid >> 6;
-    const uint32_t mine = (1u << P) - 1u;
+    const uint32_t mine = s_ok;
     const uint32_t ng = (m + 63) / 64;
     for (uint32_t g0 = (uint32_t)(wv % WPP) * 64; g0 < ng; g0 += (uint32_t)WPP * 64) {
       const uint32_t g = g0 + (uint32_t)lane;
@@ -1137,7 +1164,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   // no column outside Z, is skipped before its row is read.
 #pragma unroll 1
   for (int pp = 0; pp < P; ++pp) {
-    if (!s_any[pp]) continue;
+    if (!s_any[pp] || !((s_ok >> pp) & 1u)) continue;
     const uint32_t z = s_zm[pp];
     const uint2 pcp = codes[s_pt[pp]];
     for (int k = tid; k < W; k += NT) {
@@ -1198,7 +1225,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) {
-    if (base + pp >= m) break;
+    if (!((s_ok >> pp) & 1u)) continue;
     const uint32_t row = pos[s_pt[pp]];
     // streaming stores: the bitmaps are not read again here, and keeping
     // them out of L2 leaves it to the candidate codes the other workgroups
@@ -1209,15 +1236,31 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
 
-template <int D, int P, int NT, int ORDER>
+template <int D, int P, int NT, int ORDER, int ZB = zbits<D>()>
 __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                const uint2* __restrict__ codes,
                                                const uint32_t* __restrict__ pos, uint32_t m,
                                                int max_d, const uint32_t* __restrict__ minkey,
                                                uint32_t* __restrict__ dom, uint32_t cover,
                                                CandIndex ix) {
-  member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover, ORDER,
-                        ix);
+  member_body<D, P, NT, ZB>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover,
+                            ORDER, ix);
+}
+
+// The points a k_member launch with a narrower boundary table deferred:
+// list[0] of them at list[1 ..], P per workgroup, the grid looping over the
+// list (its length is known on the device only).
+template <int D, int P, int NT, int ORDER>
+__global__ __launch_bounds__(NT) void k_member_deferred(
+    const float* __restrict__ erows, const uint2* __restrict__ codes,
+    const uint32_t* __restrict__ pos, uint32_t m, int max_d, const uint32_t* __restrict__ minkey,
+    uint32_t* __restrict__ dom, uint32_t cover, const uint32_t* __restrict__ list, CandIndex ix) {
+  const uint32_t n = list[0];
+  ix.pts = list + 1;
+  ix.npts = n;
+  ix.defer = nullptr;
+  for (uint32_t base = blockIdx.x * P; base < n; base += gridDim.x * P)
+    member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, base, cover, ORDER, ix);
 }
 
 // ---------------------------------------------------------------------------

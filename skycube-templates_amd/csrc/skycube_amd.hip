@@ -85,7 +85,7 @@ struct sky_ctx {
   DevBuf rows, key_a, key_b, pid_a, pid_b, srows, alive, s1idx, s1rows, s1keys, s1pid,
       status, flag, ext_pids, erows, dom, cub_tmp, counter, cand, eidx, epid, pos, codes, piv,
       masks, hc_cnt, hc_off, hc_key_a, hc_key_b, hc_val_a, hc_val_b, hc_ukey, hc_runs, hc_gw,
-      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff, bcnt, gsum;
+      hc_gv, hc_goff, bmask, biota, bidx, bkey, bcode, boff, bcnt, gsum, defer;
   // the last batch
   BatchState batch;
   // membership candidate order: -1 by bucket size, 0 scan, 1 bucket index
@@ -446,6 +446,37 @@ int member_groups(sky_ctx* c, uint32_t ne) {
   return SKY_OK;
 }
 
+// The membership launch of the single-instance path.  Where the kernel runs
+// with a narrower boundary table (d >= 15), the points it defers follow in a
+// second launch with the full table.
+template <int D, int PP, int NT, int O>
+int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minkey,
+                  uint32_t cover, sky::CandIndex ix) {
+  const dim3 grid((ne + PP - 1) / PP), blk(NT);
+  constexpr int ZN = sky::narrow_zbits<D>();
+  if constexpr (ZN < sky::zbits<D>()) {
+    TRY(ensure(c, c->defer, sizeof(uint32_t) * ((size_t)ne + 1)));
+    HIPCHK(hipMemsetAsync(c->defer.p, 0, sizeof(uint32_t), c->stream));
+    ix.defer = P<uint32_t>(c->defer);
+    hipLaunchKernelGGL((sky::k_member<D, PP, NT, O, ZN>), grid, blk, 0, c->stream,
+                       P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                       (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+    LAUNCHCHK();
+    ix.defer = nullptr;
+    const dim3 grid2(std::min<uint32_t>((ne + PP - 1) / PP, 512u));
+    hipLaunchKernelGGL((sky::k_member_deferred<D, PP, NT, O>), grid2, blk, 0, c->stream,
+                       P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                       (int)max_d, minkey, P<uint32_t>(c->dom), cover, P<uint32_t>(c->defer), ix);
+    LAUNCHCHK();
+  } else {
+    hipLaunchKernelGGL((sky::k_member<D, PP, NT, O>), grid, blk, 0, c->stream,
+                       P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                       (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+    LAUNCHCHK();
+  }
+  return SKY_OK;
+}
+
 int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d,
              uint32_t* n_ext) {
   c->has_mdmc = false;
@@ -541,20 +572,12 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
         ix.gcode = P<uint2>(c->bcode);
         ix.gidx = P<uint32_t>(c->bidx);
       }
-      const dim3 grid((ne + PP - 1) / PP), blk(NT);
       if (order == sky::kOrderGroup)
-        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderGroup>), grid, blk, 0, c->stream,
-                           P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                           (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+        TRY((launch_member<D, PP, NT, sky::kOrderGroup>(c, ne, max_d, minkey, cover, ix)));
       else if (order == sky::kOrderBucket)
-        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderBucket>), grid, blk, 0,
-                           c->stream, P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos),
-                           ne, (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+        TRY((launch_member<D, PP, NT, sky::kOrderBucket>(c, ne, max_d, minkey, cover, ix)));
       else
-        hipLaunchKernelGGL((sky::k_member<D, PP, NT, sky::kOrderScan>), grid, blk, 0, c->stream,
-                           P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                           (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
-      LAUNCHCHK();
+        TRY((launch_member<D, PP, NT, sky::kOrderScan>(c, ne, max_d, minkey, cover, ix)));
     });
   } else {
     TRY(mark(c, SKY_STAGE_MEMBER));
@@ -1253,7 +1276,7 @@ void free_all(sky_ctx* c) {
                     &c->hc_cnt, &c->hc_off, &c->hc_key_a, &c->hc_key_b, &c->hc_val_a,
                     &c->hc_val_b, &c->hc_ukey, &c->hc_runs, &c->hc_gw, &c->hc_gv,
                     &c->hc_goff, &c->bmask,  &c->biota,  &c->bidx,   &c->bkey,   &c->bcode,
-                    &c->boff,    &c->bcnt,   &c->gsum};
+                    &c->boff,    &c->bcnt,   &c->gsum,   &c->defer};
   for (DevBuf* b : bufs) free_buf(c, *b);
   free_batch_buffers(c);
   c->has_mdmc = c->has_hc = c->has_batch = false;

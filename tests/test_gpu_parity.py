@@ -56,6 +56,20 @@ def test_mdmc_large_ext_with_ties(ctx, d):
     assert ext.size > 2048 or d <= 4
 
 
+@pytest.mark.parametrize("max_d", [16, 8])
+def test_mdmc_d16_many_minimum_columns(ctx, max_d):
+    """d = 16 with most points at the column minimum in more than 8 columns
+    (values 0, 1, 2 with 0 likely): the membership launch defers them to its
+    second pass with the full boundary table; both passes must agree with
+    the oracle, as must the points computed in the first."""
+    rng = np.random.default_rng(16016)
+    x = rng.choice(np.array([0, 1, 2], dtype=np.float32), size=(900, 16), p=[0.72, 0.2, 0.08])
+    x[:300] += rng.random((300, 16), dtype=np.float32) * 0.5  # fewer ties in a third
+    at_min = (x == x.min(axis=0)).sum(axis=1)
+    assert (at_min > 8).sum() > 100 and (at_min <= 8).sum() > 100
+    check_mdmc(ctx, x, max_d, tag="d16-minima")
+
+
 @pytest.mark.parametrize("d,max_d", [(6, 1), (9, 3), (12, 5), (16, 4)])
 def test_mdmc_max_d(ctx, d, max_d):
     rng = np.random.default_rng(3000 + d)
