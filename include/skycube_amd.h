@@ -78,7 +78,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *                         more candidates on average, else 0
  *   SKY_OPT_COVER         lowest-norm candidates covered before the
  *                         coverage pruning starts (64 .. 65536; 0, the
- *                         default: about n_ext / 40 as a power of two in
+ *                         default: about n_ext / 32 as a power of two in
  *                         [2048, 16384] for one skycube, 2048 in a batch)
  *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
  *                         the rest of the cover is code-tested against the
@@ -86,7 +86,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (1024; 64 .. 8192)
  *   SKY_OPT_MIN_BUCKET    bucket-size threshold above (16; 1 .. 2^30)
  *   SKY_OPT_MEMBER_GRID   workgroups of the batched persistent membership
- *                         kernel (1024; 64 .. 8192)
+ *                         kernel (2048; 64 .. 8192)
  *   SKY_OPT_TILE          points per membership workgroup in the batched
  *                         path: 0 the default, 1 half of it
  *   SKY_OPT_BATCH_ORDER   candidate order of the batched membership: 2

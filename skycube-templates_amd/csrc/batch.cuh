@@ -462,8 +462,13 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
 // (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
 // hold min_bucket candidates or more on average; cover as in member_body.
-template <int D, int P, int NT>
-__global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
+// GROUPS = true compiles the code-order group visit only (goff must be set);
+// with workgroups of four waves it also holds the kernel to 64 VGPRs, so that
+// eight waves run per SIMD (with a grid of 2048 workgroups); the few values
+// that then live in scratch cost less than the doubled occupancy gains.
+template <int D, int P, int NT, bool GROUPS = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUPS && NT == 256 ? 8 : 1)))
+void k_bmember(const float* __restrict__ erows,
                                                 const uint2* __restrict__ codes,
                                                 const uint32_t* __restrict__ pos,
                                                 const uint32_t* __restrict__ eoff, uint32_t T,
@@ -494,15 +499,15 @@ __global__ __launch_bounds__(NT) void k_bmember(const float* __restrict__ erows,
     const uint32_t t = seg_of(toff, T, task);
     const uint32_t e0 = eoff[t], m = eoff[t + 1] - e0;
     uint32_t g0 = 0, nb = 0;
-    if (gfirst) {
+    if (gfirst && !GROUPS) {
       g0 = gfirst[t];
       nb = gfirst[t + 1] - g0;
     }
-    const bool use_idx = gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
+    const bool use_idx = !GROUPS && gfirst && (uint64_t)m >= (uint64_t)nb * min_bucket;
     CandIndex ix;
     ix.head = head;
     int order = use_idx ? kOrderBucket : kOrderScan;
-    if (goff) {
+    if (GROUPS || goff) {
       order = kOrderGroup;
       ix.gsum = gsum + goff[t];
       ix.gcode = gcode + e0;

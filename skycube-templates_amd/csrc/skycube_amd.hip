@@ -67,7 +67,7 @@ struct Tuning {
   uint32_t cover = 0;  // candidates covered first; 0: by n_ext (single) / kCover (batched)
   uint32_t pruners = sky::kPruners;       // prefilter points
   uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
-  uint32_t grid = 1024;                   // workgroups of the persistent kernel
+  uint32_t grid = 2048;                   // workgroups of the persistent kernel
   uint32_t tile = 0;                      // 0: member_points, 1: half of it
   uint32_t order = 2;                     // batched membership: 2 code-order groups, 0 scan / buckets
   uint32_t head = 0;                      // cover head tested pair by pair (0: cover / 4)
@@ -550,12 +550,12 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       const Tuning& tu = c->tune[d];
       int order = c->opt_member_index;
       if (order < 0) order = sky::kOrderGroup;
-      // the cover grows with the extended skyline: about n_ext / 40, a power
-      // of two in [2048, 16384] (measured best on the three large workloads)
+      // the cover grows with the extended skyline: about n_ext / 32, a power
+      // of two in [2048, 16384] (measured best on the four large workloads)
       uint32_t cover = tu.cover;
       if (!cover) {
         cover = 2048;
-        while (cover < 16384 && (uint64_t)cover * 2 * 40 <= ne) cover *= 2;
+        while (cover < 16384 && (uint64_t)cover * 2 * 32 <= ne) cover *= 2;
       }
       sky::CandIndex ix;
       ix.head = tu.head;
@@ -1001,16 +1001,23 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
                                             P<uint32_t>(B.seg_b), T + 1, c->stream));
   }
   constexpr int NT = sky::member_threads<D>();
-  hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
-                     P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
-                     P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                     P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
-                     tu.min_bucket,
-                     tu.order == 2 ? P<uint32_t>(B.goff) : nullptr, P<uint2>(B.gsum),
-                     P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
-                     tu.order == 2 ? nullptr : P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
-                     P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
-                     P<uint2>(B.ibcode));
+  if (tu.order == 2)  // code-order groups for every instance
+    hipLaunchKernelGGL((sky::k_bmember<D, PP, NT, true>), dim3(tu.grid), dim3(NT), 0, c->stream,
+                       P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
+                       P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
+                       P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
+                       tu.min_bucket, P<uint32_t>(B.goff), P<uint2>(B.gsum), P<uint2>(B.ibcode),
+                       P<uint32_t>(B.iblocal), nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr);
+  else
+    hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
+                       P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
+                       P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
+                       P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
+                       tu.min_bucket, nullptr, P<uint2>(B.gsum), P<uint2>(B.ibcode),
+                       P<uint32_t>(B.iblocal), P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
+                       P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
+                       P<uint2>(B.ibcode));
   LAUNCHCHK();
   return SKY_OK;
 }
