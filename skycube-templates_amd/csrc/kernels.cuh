@@ -384,8 +384,10 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
   return st;
 }
 
+// Held to 64 VGPRs (eight waves per SIMD): at d = 15, 16 a few values then
+// live in scratch, which costs less than the lost occupancy did.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_self_idx(
+__global__ __attribute__((amdgpu_waves_per_eu(8))) __launch_bounds__(kBlock) void k_self_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
     const uint2* __restrict__ codes,
     const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bidx,
