@@ -83,7 +83,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
  *                         the rest of the cover is code-tested against the
  *                         closure of the head (0: a quarter of the cover)
- *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (1024; 64 .. 8192)
+ *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (128; 64 .. 8192)
  *   SKY_OPT_MIN_BUCKET    bucket-size threshold above (16; 1 .. 2^30)
  *   SKY_OPT_MEMBER_GRID   workgroups of the batched persistent membership
  *                         kernel (2048; 64 .. 8192)

@@ -17,7 +17,7 @@
 namespace sky {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
-constexpr uint32_t kPruners = 512;  // default: lowest-score points used as a prefilter
+constexpr uint32_t kPruners = 128;  // default: lowest-score points used as a prefilter
 constexpr int kXCap = 2048;       // tie-touched subspaces resolved per round
 constexpr int kCodePivots = 15;   // 16 quantile buckets per column, 4-bit codes
 

@@ -1379,7 +1379,7 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
       {"order", &Tuning::order, {2, 0}},
       {"tile", &Tuning::tile, {0, 1}},
       {"cover", &Tuning::cover, {512, 1024, 2048, 4096}},
-      {"pruners", &Tuning::pruners, {256, 512, 1024, 2048}},
+      {"pruners", &Tuning::pruners, {128, 256, 512, 1024}},
       {"min_bucket", &Tuning::min_bucket, {4, 16, 64, 1u << 30}},
       {"grid", &Tuning::grid, {256, 512, 1024, 2048}},
   };
