@@ -328,12 +328,19 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
       }
       lo = boff[S];
       uint32_t a = lo, z = boff[S + 1];
-      while (a < z) {
-        const uint32_t mid = (a + z) >> 1;
-        if (bkey[mid] <= kp) a = mid + 1;
-        else z = mid;
+      // a small bucket is taken whole (its candidates past the key bound
+      // fail the code or the exact test); a large one is cut at the bound
+      // by binary search, a chain of dependent loads
+      if (z - a > 32) {
+        while (a < z) {
+          const uint32_t mid = (a + z) >> 1;
+          if (bkey[mid] <= kp) a = mid + 1;
+          else z = mid;
+        }
+        cnt = a - lo;
+      } else {
+        cnt = z - lo;
       }
-      cnt = a - lo;
     }
     uint32_t inc = cnt;
 #pragma unroll

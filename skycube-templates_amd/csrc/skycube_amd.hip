@@ -463,8 +463,10 @@ int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minke
                        (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
     LAUNCHCHK();
     ix.defer = nullptr;
-    const dim3 grid2(std::min<uint32_t>((ne + PP - 1) / PP, 512u));
-    hipLaunchKernelGGL((sky::k_member_deferred<D, PP, NT, O>), grid2, blk, 0, c->stream,
+    // one point per workgroup: the deferred points are few, and their
+    // workgroups are the tail of the pass
+    const dim3 grid2(std::min<uint32_t>(ne, 512u));
+    hipLaunchKernelGGL((sky::k_member_deferred<D, 1, NT, O>), grid2, blk, 0, c->stream,
                        P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
                        (int)max_d, minkey, P<uint32_t>(c->dom), cover, P<uint32_t>(c->defer), ix);
     LAUNCHCHK();
