@@ -1213,20 +1213,42 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       __syncthreads();
       const int cnt = min(s_cnt, XCAP);
       if (cnt == 0) break;
-      for (uint32_t b = tid; b < m; b += NT) {
-        const uint2 qc = codes[b];
+      // candidate b with codes qc: the filters, then the exact test
+      auto settle = [&](uint32_t b, uint2 qc) {
         const uint32_t ub = planes_le(qc, pcp) & FULL;
-        if (!((A0[pp][ub >> 5] >> (ub & 31)) & 1u)) continue;
+        if (!((A0[pp][ub >> 5] >> (ub & 31)) & 1u)) return;
         const uint32_t ex = ~(qc.x ^ pcp.x), ey = ~(qc.y ^ pcp.y);
-        if (((ex >> 16) & ex & (ey >> 16) & ey & FULL & ~z) == 0) continue;
+        if (((ex >> 16) & ex & (ey >> 16) & ey & FULL & ~z) == 0) return;
         uint32_t lt, eq;
         pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
-        if (lt == 0 || (eq & ~z) == 0) continue;
+        if (lt == 0 || (eq & ~z) == 0) return;
         const uint32_t le = lt | eq;
         for (int e = 0; e < cnt; ++e) {
           const uint32_t s = xl[e];
           if (!(s & ~le) && (s & lt)) lds_mark(A1[pp], s);
         }
+      };
+      if (order == kOrderGroup) {
+        // a group's code bound contains each member's, and the closure of X
+        // in A0 is upward closed: a group whose bound is outside it holds
+        // no candidate that passes the first filter
+        const int wv = tid >> 6;
+        const uint32_t ng = (m + 63) / 64;
+        for (uint32_t g0 = (uint32_t)wv * 64; g0 < ng; g0 += (uint32_t)NT) {
+          const uint32_t g = g0 + (uint32_t)lane;
+          bool al = false;
+          if (g < ng) {
+            const uint32_t ubg = planes_le(ix.gsum[g], pcp) & FULL;
+            al = (A0[pp][ubg >> 5] >> (ubg & 31)) & 1u;
+          }
+          for (uint64_t live = __ballot(al); live; live &= live - 1ull) {  // uniform
+            const uint32_t j = (g0 + (uint32_t)(__ffsll((long long)live) - 1)) * 64u +
+                               (uint32_t)lane;
+            if (j < m) settle(ix.gidx[j], ix.gcode[j]);
+          }
+        }
+      } else {
+        for (uint32_t b = tid; b < m; b += NT) settle(b, codes[b]);
       }
     }
   }
