@@ -79,7 +79,7 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *   SKY_OPT_COVER         lowest-norm candidates covered before the
  *                         coverage pruning starts (64 .. 65536; 0, the
  *                         default: about n_ext / 32 as a power of two in
- *                         [2048, 16384] for one skycube, 2048 in a batch)
+ *                         [2048, 16384], per instance in a batch)
  *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
  *                         the rest of the cover is code-tested against the
  *                         closure of the head (0: a quarter of the cover)

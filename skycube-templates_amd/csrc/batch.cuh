@@ -461,7 +461,8 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // groups from goff[t] on; gcode / glocal in the group code order); else with gfirst,
 // instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
 // (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
-// hold min_bucket candidates or more on average; cover as in member_body.
+// hold min_bucket candidates or more on average; cover as in member_body
+// (0: auto_cover of each instance's extended skyline).
 // GROUPS = true compiles the code-order group visit only (goff must be set);
 // with workgroups of four waves it also holds the kernel to 64 VGPRs, so that
 // eight waves run per SIMD (with a grid of 2048 workgroups); the few values
@@ -521,7 +522,8 @@ void k_bmember(const float* __restrict__ erows,
       ix.bcode = bcode;
     }
     member_body<D, P, NT>(erows + (size_t)e0 * D, codes + e0, pos + e0, m, (int)maxd[t],
-                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P, cover, order, ix);
+                          minkey + (size_t)t * 16, dom, (task - toff[t]) * P,
+                          cover ? cover : auto_cover(m), order, ix);
   }
 }
 

@@ -64,7 +64,7 @@ struct BatchState {
 
 // Launch settings of one dimensionality (sky_ctx_set_option, autotuner).
 struct Tuning {
-  uint32_t cover = 0;  // candidates covered first; 0: by n_ext (single) / kCover (batched)
+  uint32_t cover = 0;  // candidates covered first; 0: auto_cover of each extended skyline
   uint32_t pruners = sky::kPruners;       // prefilter points
   uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
   uint32_t grid = 2048;                   // workgroups of the persistent kernel
@@ -552,13 +552,7 @@ int run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t
       const Tuning& tu = c->tune[d];
       int order = c->opt_member_index;
       if (order < 0) order = sky::kOrderGroup;
-      // the cover grows with the extended skyline: about n_ext / 32, a power
-      // of two in [2048, 16384] (measured best on the four large workloads)
-      uint32_t cover = tu.cover;
-      if (!cover) {
-        cover = 2048;
-        while (cover < 16384 && (uint64_t)cover * 2 * 32 <= ne) cover *= 2;
-      }
+      const uint32_t cover = tu.cover ? tu.cover : sky::auto_cover(ne);
       sky::CandIndex ix;
       ix.head = tu.head;
       if (order == sky::kOrderBucket) {
@@ -1007,7 +1001,7 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
     hipLaunchKernelGGL((sky::k_bmember<D, PP, NT, true>), dim3(tu.grid), dim3(NT), 0, c->stream,
                        P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                        P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                       P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
+                       P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head,
                        tu.min_bucket, P<uint32_t>(B.goff), P<uint2>(B.gsum), P<uint2>(B.ibcode),
                        P<uint32_t>(B.iblocal), nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr);
@@ -1015,7 +1009,7 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
     hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
                        P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                        P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
-                       P<uint32_t>(g.dom), cnt + 4, tu.cover ? tu.cover : sky::kCover, tu.head,
+                       P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head,
                        tu.min_bucket, nullptr, P<uint2>(B.gsum), P<uint2>(B.ibcode),
                        P<uint32_t>(B.iblocal), P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
                        P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
@@ -1380,7 +1374,7 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
   const Knob knobs[] = {
       {"order", &Tuning::order, {2, 0}},
       {"tile", &Tuning::tile, {0, 1}},
-      {"cover", &Tuning::cover, {512, 1024, 2048, 4096}},
+      {"cover", &Tuning::cover, {0, 512, 1024, 2048, 4096}},  // 0: auto_cover
       {"pruners", &Tuning::pruners, {128, 256, 512, 1024}},
       {"min_bucket", &Tuning::min_bucket, {4, 16, 64, 1u << 30}},
       {"grid", &Tuning::grid, {256, 512, 1024, 2048}},
