@@ -774,7 +774,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
-  __shared__ uint32_t s_pt[P];  // the points: base + pp of the score order, or of the code order
+  __shared__ uint32_t s_pt[P];  // the points: entry base + pp of the score order, the code order or ix.pts
   __shared__ uint32_t s_ok;     // bit pp: point pp exists and is computed here
   __shared__ int s_cnt;
   uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
@@ -981,17 +981,21 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
           jj[r] = (g0 + (uint32_t)L) * 64u + (uint32_t)lane;
           qc[r] = ix.gcode[min(jj[r], m - 1u)];
         }
-        uint32_t hit = 0;  // bit r * P + pp
+        // set bits of A0 are gathered as they are (bit r * P + pp) and the
+        // tested, existing pairs without one become the hits in one step
+        uint32_t seen = 0, tested = 0;
 #pragma unroll
         for (int r = 0; r < kLive; ++r) {
+          tested |= (jj[r] < m ? gal[r] : 0u) << (r * P);
 #pragma unroll
           for (int pp = 0; pp < P; ++pp) {
             if (!((gal[r] >> pp) & 1u)) continue;  // uniform
             const uint32_t ub = planes_le(qc[r], pc[pp]) & FULL;
             const uint32_t w = A0[pp][ub >> 5];
-            hit |= ((~(w >> (ub & 31)) & (uint32_t)(jj[r] < m)) & 1u) << (r * P + pp);
+            seen |= ((w >> (ub & 31)) & 1u) << (r * P + pp);
           }
         }
+        const uint32_t hit = tested & ~seen;
         if (__ballot(hit != 0u) == 0ull) continue;  // uniform
 #pragma unroll 1
         for (int r = 0; r < kLive; ++r) {
