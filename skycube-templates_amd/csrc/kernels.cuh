@@ -210,13 +210,15 @@ __device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
 // Columns k where code_q(k) <= code_p(k), for the bit-plane codes of k_codes
 // (a lexicographic compare of the planes, 16 columns at a time): the upper
 // halves compare planes 3:2, the lower halves planes 1:0, and the upper
-// result decides unless the upper planes are equal.
+// result decides unless the upper planes are equal.  Written as selects of
+// three inputs each, which gfx950 issues as one v_bitop3 apiece (7 VALU
+// operations in all); bits above D come out set.
 __device__ __forceinline__ uint32_t planes_le(uint2 q, uint2 p) {
-  const uint32_t le_b = ~q.y | p.y;                       // plane 2 | plane 0
-  const uint32_t le2 = (~q.x & p.x) | (~(q.x ^ p.x) & le_b);  // planes 3:2 | 1:0
-  const uint32_t ne_hi = ((q.x ^ p.x) | (q.y ^ p.y)) >> 16;   // planes 3:2 differ
-  const uint32_t hi = le2 >> 16, lo = le2 & 0xFFFFu;
-  return (ne_hi & hi) | (~ne_hi & lo);
+  const uint32_t x = q.x ^ p.x;                           // planes 3 | 1 differ
+  const uint32_t le2 = (x & p.x) | (~x & (~q.y | p.y));   // planes 3:2 | 1:0: q <= p
+  const uint32_t ne = x | (q.y ^ p.y);                    // planes 3:2 | 1:0 differ
+  // upper half: planes 3:2 where they differ, else planes 1:0 (moved up)
+  return ((ne & le2) | (~ne & (le2 << 16))) >> 16;
 }
 
 // ---------------------------------------------------------------------------
