@@ -13,6 +13,7 @@
 #include "kernels.cuh"
 
 namespace sky {
+namespace {
 
 // Segment of index i: the largest t < T with off[t] <= i (off ascending,
 // T + 1 entries); empty segments are skipped over.
@@ -72,7 +73,7 @@ __global__ void k_bscore(const float* __restrict__ rows, const uint32_t* __restr
     sum += v;
   }
   if (nan) atomicOr(err, 2u);
-  key[i] = ((uint64_t)t << 32) | order_key(sum);
+  key[i] = ((uint64_t)t << 32) | norm_key(sum);
   val[i] = i;
 }
 
@@ -355,11 +356,13 @@ __global__ void k_blocal(const uint32_t* __restrict__ ids, uint32_t cnt,
   out[j] = i - off[seg_of(off, T, i)];
 }
 
-// Code order per instance (for kOrderGroup): 64-bit plane keys of the
-// entries j < *count (all ones past it), sorted per instance segment.
+// Code order per instance (for kOrderGroup): keys of the entries j < *count
+// (all ones past it) with the instance above the top kbits bits of the
+// 4D-bit plane key, so that one sort orders every instance segment.
 template <int D>
-__global__ void k_bplane_keys(const uint2* __restrict__ codes, const uint32_t* __restrict__ count,
-                              uint32_t N, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+__global__ void k_bplane_keys(const uint2* __restrict__ codes, const uint32_t* __restrict__ eoff,
+                              uint32_t T, const uint32_t* __restrict__ count, uint32_t N, int kbits,
+                              uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   val[j] = j;
@@ -368,8 +371,10 @@ __global__ void k_bplane_keys(const uint2* __restrict__ codes, const uint32_t* _
     return;
   }
   const uint2 c = codes[j];
-  key[j] = ((uint64_t)(c.x >> 16) << (3 * D)) | ((uint64_t)(c.y >> 16) << (2 * D)) |
-           ((uint64_t)(c.x & 0xFFFFu) << D) | (uint64_t)(c.y & 0xFFFFu);
+  const uint64_t pk = ((uint64_t)(c.x >> 16) << (3 * D)) | ((uint64_t)(c.y >> 16) << (2 * D)) |
+                      ((uint64_t)(c.x & 0xFFFFu) << D) | (uint64_t)(c.y & 0xFFFFu);
+  const uint64_t t = seg_of(eoff, T, j);
+  key[j] = (kbits < 64 ? t << kbits : 0ull) | (pk >> (4 * D - kbits));
 }
 
 // Groups of 64 per instance: cnt[t] = ceil(m_t / 64), cnt[T] = 0 (for the
@@ -527,4 +532,5 @@ void k_bmember(const float* __restrict__ erows,
   }
 }
 
+}  // namespace
 }  // namespace sky

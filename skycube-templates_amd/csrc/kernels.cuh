@@ -14,16 +14,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.cuh"
+
+// Everything below has internal linkage: each translation unit of the
+// library compiles the kernels it launches.
 namespace sky {
-
-constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
-constexpr uint32_t kPruners = 128;  // default: lowest-score points used as a prefilter
-constexpr int kXCap = 2048;       // tie-touched subspaces resolved per round
-constexpr int kCodePivots = 15;   // 16 quantile buckets per column, 4-bit codes
-
-struct DimList {
-  int32_t idx[16];
-};
+namespace {
 
 // Order-preserving uint32 key of an FP32 value; -0 and +0 share a key.
 __device__ __forceinline__ uint32_t order_key(float f) {
@@ -32,10 +28,18 @@ __device__ __forceinline__ uint32_t order_key(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// Sort key of a Manhattan norm.  FP addition is monotone, so a dominator
+// never has a larger norm.  A row holding both +inf and -inf has a NaN norm;
+// it gets the key of -inf: such a row has a -inf, so do all its dominators,
+// whose norms are then -inf or NaN as well -- the order stays safe.
+__device__ __forceinline__ uint32_t norm_key(float sum) {
+  return sum == sum ? order_key(sum) : order_key(-__builtin_inff());
+}
+
 // Manhattan norm (ascending-column FP32 sum, the score MommyQ and SkyAlign
 // sort by: src/hybrid/mommyq.cpp:25-33) of each candidate over the subspace
-// columns, as a sort key.  FP addition is monotone, so a dominator never has
-// a larger key.  err bit 0: a candidate id is out of range; bit 1: NaN.
+// columns, as a sort key (norm_key).  err bit 0: a candidate id is out of
+// range; bit 1: NaN input.
 __global__ void k_score(const float* __restrict__ rows, uint32_t n_rows, uint32_t d,
                         const uint32_t* __restrict__ cand, uint32_t c, DimList dims,
                         uint32_t ds, uint32_t* __restrict__ key, uint32_t* __restrict__ pid,
@@ -58,7 +62,7 @@ __global__ void k_score(const float* __restrict__ rows, uint32_t n_rows, uint32_
     sum += v;
   }
   if (nan) atomicOr(err, 2u);
-  key[i] = order_key(sum);
+  key[i] = norm_key(sum);
   pid[i] = r;
 }
 
@@ -538,19 +542,6 @@ template <int D>
 constexpr int narrow_zbits() {
   return D == 16 ? 8 : zbits<D>();
 }
-constexpr uint32_t kCover = 2048;  // default: lowest-score candidates tested first
-// The default cover of an extended skyline of m points: it grows with m, about
-// m / 32 as a power of two in [kCover, 16384] (measured best on the four
-// large workloads).
-__host__ __device__ __forceinline__ uint32_t auto_cover(uint32_t m) {
-  uint32_t c = kCover;
-  while (c < 16384 && (uint64_t)c * 2 * 32 <= m) c *= 2;
-  return c;
-}
-// Default: k_member visits the candidates through the median-mask buckets
-// when they hold at least this many candidates on average (each bucket
-// costs one bound test and a share of a 64-lane round), else scans them.
-constexpr uint32_t kMinBucket = 16;
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB,
 // and no more than 4 from d = 11 on (measured: the points of a workgroup
@@ -761,7 +752,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             int max_d, const uint32_t* __restrict__ minkey,
                                             uint32_t* __restrict__ dom, uint32_t base,
                                             uint32_t cover, int order, const CandIndex& ix) {
-  constexpr int W = D >= 5 ? (1 << (D - 5)) : 1;
+  constexpr int W = D >= 5 ? (1 << (D >= 5 ? D - 5 : 0)) : 1;
   constexpr int NU = 1 << ZB;
   constexpr int QW = D == 16 ? 64 : 128;  // queue entries per wave
   constexpr int QCAP = (NT / 64) * QW;
@@ -1364,39 +1355,84 @@ __global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __r
   group_sum<D>(codes, gidx, g * 64, min(m, g * 64 + 64), 0u, gcode, nullptr, gsum + g);
 }
 
-// HashCube export: (word index, word value) -> point ids.
+// HashCube export: (word index, word value) -> point ids.  The bitmaps are
+// one flat array of ne * W words (point-major); a tile is 4096 consecutive
+// words, 16 per thread read as four 16-byte loads, so every wave reads 4 KB
+// of contiguous memory.  Entries are the words that are not all ones.
 
-// Words of point i that the HashCube keeps (not all ones), plus the extra
-// always-zero word for d >= 5.
-__global__ void k_hc_count(const uint32_t* __restrict__ dom, uint32_t ne, uint32_t W,
-                           uint32_t extra, uint32_t* __restrict__ cnt) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne) return;
-  uint32_t c = extra;
-  for (uint32_t j = 0; j < W; ++j) c += dom[(size_t)i * W + j] != 0xFFFFFFFFu;
-  cnt[i] = c;
+constexpr int kHcItems = 16;
+constexpr uint32_t kHcTile = 256 * kHcItems;
+
+__device__ __forceinline__ void hc_load(const uint32_t* __restrict__ dom, uint64_t total,
+                                        uint64_t k0, uint32_t (&v)[kHcItems]) {
+  if (k0 + kHcItems <= total) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* p = reinterpret_cast<const u32x4*>(dom + k0);
+#pragma unroll
+    for (int q = 0; q < kHcItems / 4; ++q) {
+      const u32x4 x = __builtin_nontemporal_load(p + q);
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kHcItems; ++r) v[r] = k0 + r < total ? dom[k0 + r] : 0xFFFFFFFFu;
+  }
 }
 
-// Entries (key = j << 32 | value, id) of point i from offset off[i].
-__global__ void k_hc_emit(const uint32_t* __restrict__ dom, const uint32_t* __restrict__ ids,
-                          uint32_t ne, uint32_t W, uint32_t extra,
-                          const uint64_t* __restrict__ off, uint64_t* __restrict__ key,
-                          uint32_t* __restrict__ val) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne) return;
-  uint64_t o = off[i];
-  const uint32_t pid = ids[i];
-  for (uint32_t j = 0; j < W; ++j) {
-    const uint32_t v = dom[(size_t)i * W + j];
-    if (v != 0xFFFFFFFFu) {
-      key[o] = ((uint64_t)j << 32) | v;
-      val[o] = pid;
-      ++o;
-    }
+// tsum[tile] = entries of the tile.
+__global__ __launch_bounds__(256) void k_hc_count(const uint32_t* __restrict__ dom, uint64_t total,
+                                                  uint32_t* __restrict__ tsum) {
+  __shared__ uint32_t red[4];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kHcTile + (uint64_t)threadIdx.x * kHcItems;
+  uint32_t v[kHcItems];
+  hc_load(dom, total, k0, v);
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kHcItems; ++r) s += v[r] != 0xFFFFFFFFu;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) tsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Entries (key = j << 32 | value, id of point i) of word k = i * W + j, in
+// ascending k from the tile's scanned offset (W = 2^wbits).
+__global__ __launch_bounds__(256) void k_hc_emit(const uint32_t* __restrict__ dom, uint64_t total,
+                                                 int wbits, const uint32_t* __restrict__ ids,
+                                                 const uint32_t* __restrict__ toff,
+                                                 uint64_t* __restrict__ key,
+                                                 uint32_t* __restrict__ val) {
+  __shared__ uint32_t red[4];
+  const uint64_t k0 = (uint64_t)blockIdx.x * kHcTile + (uint64_t)threadIdx.x * kHcItems;
+  uint32_t v[kHcItems];
+  hc_load(dom, total, k0, v);
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kHcItems; ++r) s += v[r] != 0xFFFFFFFFu;
+  // exclusive prefix of s over the workgroup
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
   }
-  if (extra) {
-    key[o] = (uint64_t)W << 32;
-    val[o] = pid;
+  if (lane == 63) red[wv] = inc;
+  __syncthreads();
+  uint32_t o = toff[blockIdx.x] + inc - s;
+  for (int w = 0; w < wv; ++w) o += red[w];
+  const uint64_t wmask = (1ull << wbits) - 1ull;
+#pragma unroll
+  for (int r = 0; r < kHcItems; ++r) {
+    if (v[r] == 0xFFFFFFFFu) continue;
+    const uint64_t k = k0 + r;
+    key[o] = ((k & wmask) << 32) | v[r];
+    val[o] = ids[k >> wbits];
+    ++o;
   }
 }
 
@@ -1409,4 +1445,5 @@ __global__ void k_hc_groups(const uint64_t* __restrict__ ukey, uint64_t ng,
   gv[g] = (uint32_t)ukey[g];
 }
 
+}  // namespace
 }  // namespace sky

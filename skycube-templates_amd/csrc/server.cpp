@@ -352,16 +352,20 @@ int sky_server_submit(sky_server* s, const float* rows, uint32_t n, uint32_t d, 
 int sky_server_wait(sky_server* s, uint64_t id, int timeout_ms) {
   if (!s) return SKY_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->all.find(id);
-  if (it == s->all.end()) return SKY_ERR_STATE;
-  Request* r = it->second.get();
-  auto done = [&] { return r->state == 2; };
+  if (s->all.find(id) == s->all.end()) return SKY_ERR_STATE;
+  // the request is looked up again after every wake-up: another thread may
+  // have taken its result (and freed it) meanwhile
+  auto done = [&] {
+    auto it = s->all.find(id);
+    return it == s->all.end() || it->second->state == 2;
+  };
   if (timeout_ms < 0) {
     s->cv_done.wait(lk, done);
   } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) {
     return SKY_ERR_STATE;
   }
-  return r->rc;
+  auto it = s->all.find(id);
+  return it == s->all.end() ? SKY_ERR_STATE : it->second->rc;
 }
 
 int sky_server_result(sky_server* s, uint64_t id, uint32_t* ext_pids, uint32_t* n_ext,

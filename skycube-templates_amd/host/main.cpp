@@ -7,7 +7,9 @@
 // reference (src/main.h:324-346).  -c checks the result: the reference
 // compares against its STSC template; here MDMC and SDSC, two independent
 // device algorithms, are compared cuboid by cuboid (sizes and members), and
-// each mismatch is printed.
+// each mismatch is printed.  -w <file> (not a reference flag) writes the
+// result after the timed window: the HashCube maps of mdmc or the lattice's
+// sky / ext lists of sdsc, as little-endian uint32 records (write_result).
 #include <getopt.h>
 #include <unistd.h>
 
@@ -36,8 +38,48 @@ static void printUsage() {
   printf(" -g: gpu device numbers to use in a comma separated list (the first is used)\n");
   printf(" -m: maximum amount of dimensions in each subspace of result\n");
   printf(" -i: custom papi counters (not supported: use rocprofv3)\n");
-  printf(" -o: print work distribution for cross device computation\n\n");
+  printf(" -o: print work distribution for cross device computation\n");
+  printf(" -w: write the result to a file (HashCube maps or lattice lists)\n\n");
   printf("Example: ./skycube -f ../workloads/house.csv -s mdmc\n\n");
+}
+
+// Result file of -w: "SKYC", d, kind (0 HashCube, 1 lattice), then for a
+// HashCube the number of maps and per map its group count and per group
+// (value, id count, ids); for a lattice, per cuboid s = 1 .. 2^d - 1, the
+// sky count and ids, then the ext count and ids.
+static void put(std::FILE* f, uint32_t v) { std::fwrite(&v, sizeof(v), 1, f); }
+
+static bool write_hashcube(const std::string& path, uint32_t d, HashCubeMaps* maps, size_t words) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  std::fwrite("SKYC", 1, 4, f);
+  put(f, d);
+  put(f, 0);
+  put(f, (uint32_t)words);
+  for (size_t j = 0; j < words; ++j) {
+    put(f, (uint32_t)maps[j].size());
+    for (const auto& kv : maps[j]) {
+      put(f, kv.first);
+      put(f, (uint32_t)kv.second.size());
+      for (int id : kv.second) put(f, (uint32_t)id);
+    }
+  }
+  return std::fclose(f) == 0;
+}
+
+static bool write_lattice(const std::string& path, uint32_t d, const lattice_node* lat) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  std::fwrite("SKYC", 1, 4, f);
+  put(f, d);
+  put(f, 1);
+  for (uint32_t s = 1; s < (1u << d); ++s) {
+    put(f, (uint32_t)lat[s].skyline.size());
+    for (unsigned id : lat[s].skyline) put(f, id);
+    put(f, (uint32_t)lat[s].extended_sky.size());
+    for (unsigned id : lat[s].extended_sky) put(f, id);
+  }
+  return std::fclose(f) == 0;
 }
 
 static void int_split(const char* s, std::vector<int>* out) {
@@ -54,9 +96,10 @@ static uint64_t now_ms() {
 
 int main(int argc, char** argv) {
   Config cfg;
+  std::string out_path;
   int c = 0;
   opterr = 0;
-  while ((c = getopt(argc, argv, "f:t:s:a:d:q:g:m:i:pco")) != -1) {
+  while ((c = getopt(argc, argv, "f:t:s:a:d:q:g:m:i:pcow:")) != -1) {
     switch (c) {
       case 'f': cfg.input_fname = optarg; break;
       case 't': cfg.threads = atoi(optarg); break;
@@ -70,6 +113,7 @@ int main(int argc, char** argv) {
       case 'c': cfg.check = true; break;
       case 'o': cfg.cross_distribution = true; break;
       case 'i': cfg.papi = optarg; break;
+      case 'w': out_path = optarg; break;
       default:
         if (isprint(optopt)) fprintf(stderr, "Unknown option `-%c'.\n", optopt);
         printUsage();
@@ -105,6 +149,9 @@ int main(int argc, char** argv) {
       sky.Execute();
       q_end = now_ms();
       if (cfg.cross_distribution) printf("GPU%d\n1.000000\n", first_device(cfg));
+      if (!out_path.empty() &&
+          !write_hashcube(out_path, dd, sky.getHashCubeStructure(), sky.words()))
+        throw SkyError(SKY_ERR_ARG, "cannot write " + out_path);
       if (cfg.check) {
         SDSC ref(cfg, n, dd);
         ref.Init(rows);
@@ -128,6 +175,8 @@ int main(int argc, char** argv) {
       sky.Execute();
       q_end = now_ms();
       if (cfg.cross_distribution) printf("GPU%d\n1.000000\n", first_device(cfg));
+      if (!out_path.empty() && !write_lattice(out_path, dd, sky.getLattice()))
+        throw SkyError(SKY_ERR_ARG, "cannot write " + out_path);
       if (cfg.check) {
         MDMC ref(cfg, n, dd);
         ref.Init(rows);

@@ -2,6 +2,10 @@
 
   python tests/golden/make_golden.py            NBA + seeded samples
   python tests/golden/make_golden.py --full     + full-size workload digests
+  python tests/golden/make_golden.py --full-only NAME [NAME ...]
+                                                only the full-size digests of
+                                                the named workloads (kept
+                                                entries of the others stay)
   python tests/golden/make_golden.py --copy-nba copies the NBA input from the
                                                 reference checkout (once)
 
@@ -128,9 +132,11 @@ def make_samples() -> dict:
     return meta
 
 
-def make_full() -> dict:
+def make_full(names=None) -> dict:
     meta = {}
     for wl, w in datagen.WORKLOADS.items():
+        if names and wl not in names:
+            continue
         t = time.time()
         x = datagen.workload(wl)
         ext, dom = skyorc.mdmc(x, w["max_d"])
@@ -153,6 +159,7 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--copy-nba", action="store_true")
+    ap.add_argument("--full-only", nargs="+", metavar="NAME")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     if args.copy_nba:
@@ -160,6 +167,11 @@ def main() -> None:
             shutil.copyfileobj(src, dst)
     path = os.path.join(HERE, "golden.json")
     meta = json.load(open(path)) if os.path.exists(path) else {}
+    if args.full_only:
+        meta.setdefault("full", {}).update(make_full(args.full_only))
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        return
     meta["nba"] = make_nba()
     meta["samples"] = make_samples()
     if args.full:

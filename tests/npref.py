@@ -52,6 +52,25 @@ def brute(x: np.ndarray):
     return dom, sdom
 
 
+def brute_rows(x: np.ndarray):
+    """The same as brute(), one point at a time (memory O(n * 2^d)), for
+    d up to 16."""
+    x = np.asarray(x, dtype=np.float32)
+    n, d = x.shape
+    S = 1 << d
+    w = (1 << np.arange(d)).astype(np.int64)
+    s = np.arange(S, dtype=np.int64)[None, :]
+    dom = np.zeros((n, S), dtype=bool)
+    sdom = np.zeros((n, S), dtype=bool)
+    for i in range(n):
+        others = np.delete(x, i, axis=0)
+        lt = ((others < x[i]) * w).sum(axis=1)[:, None]
+        le = ((others <= x[i]) * w).sum(axis=1)[:, None]
+        dom[i] = (((s & ~le) == 0) & ((s & lt) != 0)).any(axis=0)
+        sdom[i] = (((s & ~lt) == 0) & (s != 0)).any(axis=0)
+    return dom, sdom
+
+
 def scope(d: int, max_d: int) -> np.ndarray:
     s = np.arange(1 << d)
     pc = np.array([bin(v).count("1") for v in s])

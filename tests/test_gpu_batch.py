@@ -108,3 +108,37 @@ def test_batch_autotune(ctx):
         got = c.batch_mdmc(mats, use_graph=True)
         for k, ((e1, d1), (e2, d2)) in enumerate(zip(ref, got)):
             assert np.array_equal(e1, e2) and np.array_equal(d1, d2), k
+
+
+def test_batch_wide_and_narrow_d(ctx):
+    """d = 1 and d = 13..16 through the batched kernels (512-thread
+    workgroups, the full boundary table), with a d = 16 tie-heavy instance
+    whose points mostly hold more than 8 column minima: every instance
+    equals the single path, a third equal the oracle, under both tile widths
+    and both candidate orders."""
+    import skycube_amd
+    rng = np.random.default_rng(31)
+    mats = []
+    for d in (1, 13, 14, 15, 16, 1, 16):
+        n = int(rng.integers(200, 1500))
+        mats.append(rng.integers(0, 5, size=(n, d)).astype(np.float32))
+        mats.append(rng.random((n, d), dtype=np.float32))
+    x = rng.choice(np.array([0, 1, 2], dtype=np.float32), size=(700, 16), p=[0.72, 0.2, 0.08])
+    assert ((x == x.min(axis=0)).sum(axis=1) > 8).mean() > 0.5
+    mats.append(x)
+    mats.append(datagen.anticorrelated(2500, 14, 77))
+    caps = [None if k % 3 else max(1, m.shape[1] - 5) for k, m in enumerate(mats)]
+    alone = [ctx.mdmc(m, caps[k]) for k, m in enumerate(mats)]
+    for k in range(0, len(mats), 3):
+        print(f"oracle instance {k}: {mats[k].shape}", flush=True)
+        oe, od = skyorc.mdmc(mats[k], caps[k] or mats[k].shape[1])
+        assert np.array_equal(alone[k][0], oe) and np.array_equal(alone[k][1], od), k
+    for tile in (0, 1):
+        for order in (0, 2):
+            with skycube_amd.Context(0) as c:
+                c.set_option(skycube_amd.SKY_OPT_TILE, tile)
+                c.set_option(skycube_amd.SKY_OPT_BATCH_ORDER, order)
+                print(f"batch tile={tile} order={order}", flush=True)
+                got = c.batch_mdmc(mats, [m or 0 for m in caps])
+                for k, ((e1, d1), (e2, d2)) in enumerate(zip(alone, got)):
+                    assert np.array_equal(e1, e2) and np.array_equal(d1, d2), (tile, order, k)

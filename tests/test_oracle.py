@@ -31,6 +31,35 @@ CASES = [(n, d, lv, seed) for seed, (n, d, lv) in enumerate(
      (33, 6, 5), (24, 7, 3), (20, 8, 2), (60, 3, 1000), (50, 6, 1000)])]
 
 
+# d = 9..16: fewer points, every pair still decides every subspace
+WIDE = [(n, d, lv, 40 + seed) for seed, (n, d, lv) in enumerate(
+    [(24, 9, 3), (22, 10, 2), (20, 11, 3), (18, 12, 2), (18, 13, 3), (16, 14, 2),
+     (16, 15, 3), (16, 16, 2), (12, 16, 1000), (20, 16, 3)])]
+
+
+@pytest.mark.parametrize("n,d,levels,seed", WIDE)
+@pytest.mark.parametrize("cap", [None, 3, 8])
+def test_wide_d_match_numpy(n, d, levels, seed, cap):
+    """Brute force, the STSC walk and the per-point closure against the
+    numpy statement at d = 9..16 (the word-level closures of the oracle act
+    from d = 6 on, its 2^8-entry tables from d = 9 on), with and without -m."""
+    x = npref.tie_heavy(np.random.default_rng(seed), n, d, levels)
+    max_d = d if cap is None else min(cap, d)
+    rdom, rsdom = npref.brute_rows(x)
+    S = 1 << d
+    m = npref.scope(d, max_d)
+    dom, sdom = skyorc.brute(x)
+    assert np.array_equal(npref.unpack_bits(dom, S)[:, 1:], rdom[:, 1:])
+    assert np.array_equal(npref.unpack_bits(sdom, S)[:, 1:], rsdom[:, 1:])
+    st_dom, st_sdom, _, _ = skyorc.stsc(x, max_d)
+    assert np.array_equal(npref.unpack_bits(st_dom, S)[:, m], rdom[:, m])
+    assert np.array_equal(npref.unpack_bits(st_sdom, S)[:, m], rsdom[:, m])
+    ext, mdom = skyorc.mdmc(x, max_d)
+    assert np.array_equal(ext, np.flatnonzero(~rsdom[:, S - 1]))
+    assert np.array_equal(npref.unpack_bits(mdom, S)[:, m], rdom[ext][:, m])
+    assert npref.unpack_bits(mdom, S)[:, ~m].all()
+
+
 @pytest.mark.parametrize("n,d,levels,seed", CASES)
 def test_brute_matches_numpy(n, d, levels, seed):
     x = npref.tie_heavy(np.random.default_rng(seed), n, d, levels)
