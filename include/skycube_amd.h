@@ -150,6 +150,26 @@ int sky_mdmc(sky_ctx *ctx, const float *rows, uint32_t n, uint32_t d,
 int sky_mdmc_device(sky_ctx *ctx, const float *rows_dev, uint32_t n, uint32_t d,
                     uint32_t max_d, uint32_t *n_ext);
 
+/* sky_mdmc_device for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No reference counterpart.  It calls no hipMalloc, hipFree or allocating
+ * library routine: the caller passes the rows and the outputs in device
+ * memory -- ext_pids_dev (capacity n ids) and dom_words_dev (capacity
+ * dom_cap_words >= n * sky_words_per_point(d)) receive the same ids and
+ * bitmaps sky_mdmc_device would leave on the device, *n_ext their count --
+ * and every other buffer is carved from a 64 MiB scratch array that the
+ * library holds on each device until the process ends.  Sizes with
+ * n * (136 + 12 d) + 2^(d+2) + 2 MiB above 64 MiB are rejected with
+ * SKY_ERR_CAPACITY (at d = 16 about 197,000 points, at d = 8 about 280,000).
+ * The context supplies only its stream; its own buffers and held results are
+ * left alone, and no result is held afterwards (sky_mdmc_fetch / _hashcube
+ * report SKY_ERR_STATE).  All calls on one device share its scratch array:
+ * calls that overlap on one device (from several contexts or threads) race. */
+int sky_mdmc_device_noalloc(sky_ctx *ctx, const float *rows_dev, uint32_t n, uint32_t d,
+                            uint32_t max_d, uint32_t *ext_pids_dev,
+                            uint32_t *dom_words_dev, uint64_t dom_cap_words,
+                            uint32_t *n_ext);
+
 /* Copies the bitmaps (and ext pids, if ext_pids is not NULL) of the last
  * sky_mdmc / sky_mdmc_device call to host memory. */
 int sky_mdmc_fetch(sky_ctx *ctx, uint32_t *ext_pids, uint32_t *dom_words,

@@ -661,6 +661,9 @@ int sky_ctx_create(int device, size_t hbm_budget_bytes, sky_ctx** out) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : free_b;
+  // the allocation-free MDMC's scratch address on this device (once); a
+  // failure only disables sky_mdmc_device_noalloc, which then reports it
+  (void)noalloc_init_device(device);
   *out = c;
   return SKY_OK;
 }

@@ -47,6 +47,7 @@ EXPORTS = (
     "sky_batch_fetch_into", "sky_ctx_get_option", "sky_batch_autotune",
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
+    "sky_mdmc_device_noalloc",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -89,6 +90,9 @@ def load():
                            ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.c_uint64]
     L.sky_mdmc_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                   ctypes.c_uint32, ctypes.c_uint32, _u32p]
+    L.sky_mdmc_device_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_uint64, _u32p]
     L.sky_mdmc_fetch.argtypes = [ctypes.c_void_p, _u32p, _u32p, ctypes.c_uint64]
     L.sky_mdmc_device_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                           ctypes.POINTER(ctypes.c_void_p)]
@@ -303,6 +307,17 @@ class Context:
         ne = np.zeros(1, dtype=np.uint32)
         self._check(self._L.sky_mdmc_device(self._h, ctypes.c_void_p(rows_dev_ptr), n, d,
                                             0 if max_d is None else max_d, _ptr(ne, _u32p)))
+        return int(ne[0])
+
+    def mdmc_device_noalloc(self, rows_dev_ptr: int, n: int, d: int, max_d: int | None,
+                            ext_dev_ptr: int, dom_dev_ptr: int, dom_cap_words: int) -> int:
+        """sky_mdmc_device_noalloc: rows, ids and bitmaps all in caller-owned
+        device memory; nothing is allocated.  Returns n_ext."""
+        ne = np.zeros(1, dtype=np.uint32)
+        self._check(self._L.sky_mdmc_device_noalloc(
+            self._h, ctypes.c_void_p(rows_dev_ptr), n, d, 0 if max_d is None else max_d,
+            ctypes.c_void_p(ext_dev_ptr), ctypes.c_void_p(dom_dev_ptr), dom_cap_words,
+            _ptr(ne, _u32p)))
         return int(ne[0])
 
     def mdmc_fetch(self, n_ext: int, d: int):
