@@ -1,18 +1,35 @@
-"""Benchmark: one skycube per step on one MI355X.
+"""Benchmark of the skycube engine on one MI355X: one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                  [--no-cpu-baseline] [--verify]
+                  [--no-batch] [--no-serve] [--no-sdsc] [--no-cpu-baseline]
 
-A step is one full skycube (MDMC: extended skyline + per-point cuboid
-membership bitmaps) of the named workload, with the input already resident
-in HBM and the result left in HBM.  Default workload: integer_ties, n =
-1,000,000 points, d = 16, subspaces of at most 8 dimensions (39,202 cuboids),
-integer values 0..255 stored as FP32 (seed 3).
+The headline (`value`) is ms per skycube of integer_ties, n = 1,000,000,
+d = 16, subspaces of at most 8 dimensions (39,202 cuboids; integers 0..255
+stored as FP32, seed 3), timed over the reference's own window
+(src/main.h:324-336: the rows already in host memory after Init, then
+Execute -- rows up to the device, the skycube, the HashCube materialised in
+host memory).  Like the reference's Init, which copies the rows into its own
+flat buffer, the rows are copied once, untimed, into pinned host memory; the
+HashCube lands in pinned host buffers.  A step is one such window; W steps
+warm up, K are timed.  The same line carries:
 
-Prints one JSON line: the headline metric (ms per skycube, lower is better),
-the roofline of the dominant kernel (membership bitmaps) from HIP events on
-the library's own stream, and the CPU baseline: the oracle port timed on
-this host's cores on a bounded sample of the same workload and scaled up.
+  stages_ms       device-only time of the skycube (rows already in HBM,
+                  bitmaps left there) and its stages, from HIP events
+  roofline        the membership kernel k_member against HBM bandwidth
+  batch           4,096 instances (n log-uniform 1k..65k, d 4..10, mixed
+                  distributions, seed 4): instances/s of the batched pass
+                  replayed from its HIP graph, against one plain MDMC call
+                  per instance on the same GPU
+  serve           600 requests (Poisson arrivals, n log-normal median 20k,
+                  d 6..14, seed 5) through the continuous-batching server
+                  at about 70% busy: p50 / p99 latency, requests/s
+  sdsc            the device SDSC lattice walk, ms, on the independent
+                  (n = 100,000, d = 8) and anticorrelated (n = 500,000,
+                  d = 12) workloads
+  cpu_baseline    the reference's own CPU path (STSC: MommyQ per cuboid,
+                  built from the reference sources into oracle/_ref) timed on
+                  this host on a bounded sample of the same workload and
+                  scaled up (see ref_cpu_baseline)
 """
 from __future__ import annotations
 
@@ -32,9 +49,10 @@ import numpy as np  # noqa: E402
 
 METRIC = "ms/skycube (n=1M,d=16) vs CPU; batched inst/s vs plain MDMC; p99 latency"
 HBM_PEAK_GBS = 8000.0
-# wave64 VALU instructions per second the chip can issue: 256 CUs x 4 SIMDs,
-# one instruction per SIMD every 4 clocks, at 2.4 GHz
-VALU_PEAK_IPS = 256 * 4 * 2.4e9 / 4
+
+
+def say(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def cuboids_in_scope(d: int, max_d: int) -> int:
@@ -49,47 +67,83 @@ def member_bytes(n_ext: int, d: int, max_d: int) -> int:
     return n_ext * (4 * d + 4 + (C + 7) // 8)
 
 
-def cpu_baseline(x: np.ndarray, max_d: int, ext: np.ndarray, target_s: float = 8.0) -> dict:
-    """The oracle port on this host: the extended skyline's prefilter in full
-    and its survivor pass on every k-th survivor, then membership bitmaps of
-    a random sample of extended-skyline points; each sampled part is scaled
-    by its sampling factor."""
-    from oracle import skyorc
+# --------------------------------------------------------------------------
+# headline: the reference's Execute() window
 
-    n, d = x.shape
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    rng = np.random.default_rng(0)
-    # membership: calibrate the sample size on a small probe first
-    probe = np.sort(rng.choice(ext.size, size=min(ext.size, 64), replace=False))
-    t = time.perf_counter()
-    skyorc.mdmc_sample(x, ext, probe, max_d)
-    per_pt = (time.perf_counter() - t) / max(probe.size, 1)
-    k = int(min(ext.size, max(64, target_s / max(per_pt, 1e-9))))
-    smp = np.sort(rng.choice(ext.size, size=k, replace=False))
-    t = time.perf_counter()
-    skyorc.mdmc_sample(x, ext, smp, max_d)
-    t_member = (time.perf_counter() - t) * ext.size / k
-    # extended skyline: prefilter in full, survivor pass sampled
-    stride = max(1, int(ext.size / max(1, k)))
-    pr, se, ns = skyorc.ext_skyline_timed(x, stride)
-    t_ext = pr + se * stride
-    total_ms = (t_ext + t_member) * 1e3
-    return {
-        "value": round(total_ms, 1), "unit": "ms/skycube", "cores": threads, "kind": "port",
-        "sample": (f"CPU oracle (C, OpenMP) on the same input: extended-skyline prefilter over "
-                   f"all {n} points, survivor pass on every {stride}th of {ns} survivors "
-                   f"(x{stride}), membership of {k} of {ext.size} extended-skyline points "
-                   f"(x{ext.size / k:.1f}); {threads} threads"),
-        "parts_ms": {"ext_skyline": round(t_ext * 1e3, 1), "membership": round(t_member * 1e3, 1)},
-    }
+def bench_window(args, torch, skycube_amd, datagen, w, x):
+    n, d, max_d = w["n"], w["d"], w["max_d"]
+    rows_pin = torch.from_numpy(x).pin_memory()  # the rows after Init
+    ext_pin = torch.empty(n, dtype=torch.int32).pin_memory()
+    ctx = skycube_amd.Context(0)
+    say("window: warm-up")
+    ctx.mdmc_raw(rows_pin.data_ptr(), n, d, max_d, ext_pin.data_ptr())
+    ng, nt = ctx.hashcube_sizes()
+    cap_g, cap_e = ng + ng // 8 + 16, nt + nt // 8 + 16
+    gw = torch.empty(cap_g, dtype=torch.int32).pin_memory()
+    gv = torch.empty(cap_g, dtype=torch.int32).pin_memory()
+    go = torch.empty(cap_g + 1, dtype=torch.int64).pin_memory()
+    ids = torch.empty(cap_e, dtype=torch.int32).pin_memory()
+
+    def step():
+        ne = ctx.mdmc_raw(rows_pin.data_ptr(), n, d, max_d, ext_pin.data_ptr())
+        g, e = ctx.hashcube_into(gw.data_ptr(), gv.data_ptr(), go.data_ptr(), ids.data_ptr(),
+                                 cap_g, cap_e)
+        return ne, g, e
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ne, g, e = step()
+        times.append(time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    # the parts of the window, one more time with the HashCube timed alone
+    t0 = time.perf_counter()
+    ctx.mdmc_raw(rows_pin.data_ptr(), n, d, max_d, ext_pin.data_ptr())
+    t1 = time.perf_counter()
+    ctx.hashcube_into(gw.data_ptr(), gv.data_ptr(), go.data_ptr(), ids.data_ptr(), cap_g, cap_e)
+    t2 = time.perf_counter()
+    ctx.close()
+    return {"ms": float(np.mean(times)) * 1e3, "ms_min": float(np.min(times)) * 1e3,
+            "n_ext": ne, "groups": g, "entries": e,
+            "parts_ms": {"rows_up_skycube_ids_down": round((t1 - t0) * 1e3, 3),
+                         "hashcube_to_host": round((t2 - t1) * 1e3, 3)}}
 
 
-def bench_batch(args, torch, skycube_amd, datagen) -> int:
-    """Batched workload: instances of log-uniform n in [1000, 65536], d in
-    {4..10}, independent / correlated / anticorrelated (seed 4).  Timed with
-    the rows resident in HBM: (a) one plain MDMC call per instance, (b) the
-    batched pass, (c) the batched pass replayed from its HIP graph."""
+def bench_device(args, torch, skycube_amd, w, x):
+    """The skycube alone: rows resident in HBM, bitmaps left there."""
+    n, d, max_d = w["n"], w["d"], w["max_d"]
+    xd = torch.from_numpy(x).to("cuda")
+    torch.cuda.synchronize()
+    ctx = skycube_amd.Context(0)
+    ctx.set_timing(True)
+    for _ in range(max(1, args.warmup)):
+        ctx.mdmc_device(xd.data_ptr(), n, d, max_d)
+    torch.cuda.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n_ext = ctx.mdmc_device(xd.data_ptr(), n, d, max_d)
+        stages.append(ctx.stage_times())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ext, _ = ctx.mdmc_fetch(n_ext, d)
+    ctx.close()
+    st = {k: round(float(np.mean([s[k] for s in stages])), 3) for k in stages[0]}
+    return ms, n_ext, st, ext
+
+
+# --------------------------------------------------------------------------
+# batched and serving configurations
+
+def bench_batch(args, torch, skycube_amd, datagen):
+    """4,096 instances, rows resident in HBM: (a) one plain MDMC call per
+    instance, (b) the batched pass replayed from its HIP graph after the
+    autotuner (on every 8th instance) picked the settings."""
     specs = datagen.batch_specs(args.instances, seed=4)
+    say(f"batch: generating {len(specs)} instances")
     mats = [datagen.batch_instance(s) for s in specs]
     n = np.array([m.shape[0] for m in mats], dtype=np.uint32)
     d = np.array([m.shape[1] for m in mats], dtype=np.uint32)
@@ -99,71 +153,59 @@ def bench_batch(args, torch, skycube_amd, datagen) -> int:
     base = xd.data_ptr()
     foff = np.concatenate([[0], np.cumsum(n.astype(np.int64) * d)])
     ctx = skycube_amd.Context(0)
-    # autotuner on a sample of the batch (every 8th instance); its settings
-    # (per dimensionality) are then used for the batched runs below
     sample = np.arange(0, len(mats), 8)
     sflat = np.concatenate([mats[k].ravel() for k in sample])
     t0 = time.perf_counter()
     tuned = ctx.batch_autotune_raw(sflat.ctypes.data, False, n[sample], d[sample])
     t_tune = time.perf_counter() - t0
-    plain_ctx = skycube_amd.Context(0)  # default settings for the plain path
-    # (a) plain: one MDMC per instance, rows already on the device
+    plain_ctx = skycube_amd.Context(0)
     for t in range(min(8, len(mats))):
         plain_ctx.mdmc_device(base + 4 * int(foff[t]), int(n[t]), int(d[t]))
     torch.cuda.synchronize()
+    say("batch: plain calls")
     t0 = time.perf_counter()
-    plain_ext = 0
     for t in range(len(mats)):
-        plain_ext += plain_ctx.mdmc_device(base + 4 * int(foff[t]), int(n[t]), int(d[t]))
+        plain_ctx.mdmc_device(base + 4 * int(foff[t]), int(n[t]), int(d[t]))
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     plain_ctx.close()
-    # (b) batched, eager; (c) batched, graph replay
     res = {}
     for label, graph in (("eager", False), ("graph", True)):
-        for _ in range(max(1, args.warmup)):
-            ctx.batch_mdmc_raw(base, True, n, d, None, graph)
+        say(f"batch: {label}")
+        ctx.batch_mdmc_raw(base, True, n, d, None, graph)
         torch.cuda.synchronize()
         times = []
-        for _ in range(max(1, args.steps)):
+        for _ in range(args.batch_steps):
             t0 = time.perf_counter()
             tot = ctx.batch_mdmc_raw(base, True, n, d, None, graph)
             times.append(time.perf_counter() - t0)
         res[label] = (float(np.median(times)), tot)
     ctx.close()
     ips = {k: len(mats) / v[0] for k, v in res.items()}
-    out = {
-        "metric": METRIC, "value": round(ips["graph"], 1), "unit": "instances/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(res["graph"][0] * 1e3, 3), "higher_is_better": True,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"batched: {len(mats)} instances, n log-uniform [1000, 65536], "
-                               f"d in 4..10, mixed distributions, seed 4",
-                   "points": int(n.sum()), "ext_points": res["graph"][1]},
+    return {
+        "value": round(ips["graph"], 1), "unit": "instances/s",
+        "workload": (f"{len(mats)} instances, n log-uniform [1000, 65536], d in 4..10, "
+                     f"independent / correlated / anticorrelated, seed 4, rows resident in HBM"),
+        "points": int(n.sum()), "ext_points": res["graph"][1],
+        "ms_per_batch": round(res["graph"][0] * 1e3, 2),
         "plain_mdmc": {"instances_per_s": round(len(mats) / t_plain, 1),
-                       "ms_total": round(t_plain * 1e3, 1), "ext_points": plain_ext},
+                       "ms_total": round(t_plain * 1e3, 1)},
         "batched_eager": {"instances_per_s": round(ips["eager"], 1),
                           "ms_total": round(res["eager"][0] * 1e3, 1)},
-        "batched_graph": {"instances_per_s": round(ips["graph"], 1),
-                          "ms_total": round(res["graph"][0] * 1e3, 1)},
         "speedup_vs_plain": round(ips["graph"] * t_plain / len(mats), 2),
         "autotune": {"sample": f"every 8th instance ({sample.size})",
                      "seconds": round(t_tune, 2), "per_d": tuned},
     }
-    print(json.dumps(out), flush=True)
-    return 0
 
 
 def _pct(a, q):
-    """q-th percentile of a, rounded (None if empty)."""
     return round(float(np.percentile(np.asarray(a), q)), 3) if len(a) else None
 
 
 def _serve_stream(srv, mats, stream, keep_every):
-    """Submits mats[k] at stream[k]'s arrival time from this thread while a
-    collector thread takes the results in order.  Returns per-request
-    latency (arrival -> result in host memory, ms), batch sizes, the kept
-    results, wall seconds and errors."""
+    """Submits mats[k] at stream[k]'s arrival time while a collector thread
+    takes the results in order: per-request latency (arrival -> result in
+    host memory, ms), batch sizes, kept results, wall seconds, errors."""
     import threading
 
     count = len(mats)
@@ -198,22 +240,19 @@ def _serve_stream(srv, mats, stream, keep_every):
     return lat, bsz, kept, time.perf_counter() - t0, errors
 
 
-def bench_serve(args, torch, skycube_amd, datagen) -> int:
-    """Serving workload: a Poisson stream of skycube requests, n log-normal
-    (median 20,000, sigma 1.0, clipped to [500, 500,000]), d in 6..14, mixed
-    distributions (seed 5), submitted from the host at their arrival times
-    to the continuous-batching server.  The arrival rate is searched (at most
-    three runs) so that the server is busy about 70% of the time, unless
-    --qps is given.  Latency = arrival -> result in host memory."""
+def bench_serve(args, torch, skycube_amd, datagen):
+    """600 requests submitted from the host at Poisson arrival times to the
+    continuous-batching server; the rate is searched (at most three runs) so
+    that the server is busy about 70% of the time.  Latency = arrival ->
+    result in host memory.  Every 10th result of the last run is checked
+    against the same rows computed alone."""
     count = args.requests
     sizes = datagen.serving_stream(count, 1.0, seed=5)
+    say(f"serve: generating {count} requests")
     mats = [datagen.by_name(dist, n, d, sd) for (_, n, d, dist, sd) in sizes]
-    pts = int(sum(m.shape[0] for m in mats))
     srv = skycube_amd.Server(0, hbm_budget=0, workers=args.workers)
-    # warm-up: one closed-loop pass (kernels, device buffers, pinned slabs)
-    for i in [srv.submit(m) for m in mats]:
+    for i in [srv.submit(m) for m in mats]:  # warm-up pass
         srv.result(i)
-    # closed loop: every request submitted at once
     t0 = time.perf_counter()
     ids = [srv.submit(m) for m in mats]
     for i in ids:
@@ -225,50 +264,124 @@ def bench_serve(args, torch, skycube_amd, datagen) -> int:
     for attempt in range(1 if args.qps else 3):
         stream = datagen.serving_stream(count, qps, seed=5)
         st0 = srv.stats()
-        lat, bsz, kept, t_open, errors = _serve_stream(srv, mats, stream, args.verify_every)
+        lat, bsz, kept, t_open, errors = _serve_stream(srv, mats, stream, 10)
         st1 = srv.stats()
         busy = (st1["busy_ms"] - st0["busy_ms"]) / (t_open * 1e3)
         runs.append({"qps": round(qps, 1), "busy": round(busy, 3)})
-        print(json.dumps({"serve_run": attempt, "offered_qps": round(qps, 1),
-                          "server_busy_frac": round(busy, 3)}), file=sys.stderr, flush=True)
+        say(f"serve: run {attempt} at {qps:.0f}/s, busy {busy:.2f}")
         if args.qps or 0.6 <= busy <= 0.8 or errors:
             break
         qps = qps * min(4.0, 0.7 / max(busy, 0.05))
     srv.close()
-    # the sampled results of the last run, each recomputed alone on a plain
-    # context, must match
-    verified = None
-    if kept:
-        ctx = skycube_amd.Context(0)
-        ok = 0
+    ok = 0
+    with skycube_amd.Context(0) as ctx:
         for k, (e0, d0) in kept.items():
             e, dm = ctx.mdmc(mats[k])
             ok += int(np.array_equal(e, e0) and np.array_equal(dm, d0))
-        ctx.close()
-        verified = f"{ok}/{len(kept)} sampled requests identical to the same rows computed alone"
     good = [v for v in lat if v is not None]
-    out = {
-        "metric": METRIC, "value": _pct(good, 99), "unit": "ms (p99 latency)", "n_gpus": 1,
-        "steps": count, "warmup": count, "ms_per_step": round(t_open * 1e3 / count, 3),
-        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"serving: {count} requests, Poisson arrivals at {qps:.1f}/s, "
-                               f"n log-normal (median 20000, sigma 1.0, [500, 500000]), "
-                               f"d in 6..14, mixed distributions, seed 5",
-                   "points": pts, "workers": args.workers},
+    return {
+        "value": _pct(good, 99), "unit": "ms (p99 latency)",
+        "workload": (f"{count} requests, Poisson arrivals, n log-normal (median 20000, sigma 1.0, "
+                     f"[500, 500000]), d in 6..14, mixed distributions, seed 5"),
         "latency_ms": {"p50": _pct(good, 50), "p90": _pct(good, 90), "p99": _pct(good, 99),
                        "max": _pct(good, 100), "mean": round(float(np.mean(good)), 3)},
-        "requests_per_s": round(count / t_open, 1),
-        "closed_loop_capacity_per_s": round(capacity, 1),
-        "offered_qps": round(qps, 1),
-        "server_busy_frac": runs[-1]["busy"],
-        "rate_search": runs,
+        "requests_per_s": round(count / t_open, 1), "offered_qps": round(qps, 1),
+        "closed_loop_capacity_per_s": round(capacity, 1), "server_busy_frac": runs[-1]["busy"],
+        "rate_search": runs, "workers": args.workers,
         "batch_size": {"mean": round(float(np.mean(bsz)), 2), "max": int(max(bsz))},
-        "verified": verified, "errors": errors or None,
+        "verified_alone": f"{ok}/{len(kept)} sampled requests identical to the same rows alone",
+        "errors": errors or None,
     }
-    print(json.dumps(out), flush=True)
-    return 0 if not errors else 1
 
+
+def bench_sdsc(args, torch, skycube_amd, datagen):
+    """The device SDSC walk (rows resident in HBM, lattice left there)."""
+    out = {}
+    with skycube_amd.Context(0) as ctx:
+        for wl in ("independent_distinct", "anticorrelated_plane"):
+            w = datagen.WORKLOADS[wl]
+            x = datagen.workload(wl)
+            xd = torch.from_numpy(x).to("cuda")
+            torch.cuda.synchronize()
+            say(f"sdsc: {wl}")
+            ctx.sdsc_device(xd.data_ptr(), w["n"], w["d"], w["max_d"])  # warm-up
+            t0 = time.perf_counter()
+            ctx.sdsc_device(xd.data_ptr(), w["n"], w["d"], w["max_d"])
+            torch.cuda.synchronize()
+            out[wl] = {"ms": round((time.perf_counter() - t0) * 1e3, 2),
+                       "cuboids": cuboids_in_scope(w["d"], w["max_d"]),
+                       "n": w["n"], "d": w["d"]}
+            del xd
+    return out
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the reference's STSC (MommyQ per cuboid), sampled
+
+def ref_cpu_baseline(skycube_amd, x, max_d, threads, per_level=None):
+    """The reference's CPU path (its STSC walk: MommyQ per cuboid, from the
+    reference sources in oracle/_ref) timed on this host with `threads`
+    threads, on a sample, scaled up:
+
+      * the full space over all points, in full (MommyQ with every thread);
+      * for each level L <= max_d, `threads` cuboids drawn at random, run
+        together as the reference runs a level (one thread per cuboid), each
+        over the candidate list the reference gives it: the full space's
+        sky + ext where its parents were skipped (L = max_d < d), else the
+        sky + ext of its parent with the fewest (the lists of those parents
+        come from the GPU, one SkyAlign over the full space's sky + ext per
+        parent);
+      * level time = sum of the sampled cuboids' seconds x (cuboids of the
+        level / sampled) / threads.
+    """
+    from oracle import skyref
+
+    n, d = x.shape
+    S = 1 << d
+    full = S - 1
+    rng = np.random.default_rng(0)
+    sky, ext, t_full = skyref.cuboid(x, full, None, threads=threads)
+    E = np.sort(np.concatenate([sky, ext])).astype(np.uint32)
+    xe = np.ascontiguousarray(x[E])
+    parts = {"full_space_s": round(t_full, 3)}
+    total = t_full
+    sample_wall = t_full
+    ctx = skycube_amd.Context(0)
+    memo = {}
+
+    def lists(p):  # sky + ext of cuboid p over all points, ascending ids
+        if p not in memo:
+            s_, e_ = ctx.skyalign(xe, p)
+            memo[p] = np.sort(E[np.concatenate([s_, e_])])
+        return memo[p]
+
+    for L in range(min(d - 1, max_d), 0, -1):
+        subs = [s for s in range(1, S) if bin(s).count("1") == L]
+        k = min(len(subs), per_level or threads)
+        pick = [subs[i] for i in sorted(rng.choice(len(subs), size=k, replace=False))]
+        cands = []
+        for s in pick:
+            if L + 1 > max_d and L + 1 < d:  # the parents' level was skipped
+                cands.append(E)
+                continue
+            parents = [s | (1 << j) for j in range(d - 1, -1, -1) if not (s >> j) & 1]
+            if len(parents) == 1 and parents[0] == full:
+                cands.append(E)
+                continue
+            best = min(parents, key=lambda p: (lists(p).size, -p))
+            cands.append(lists(best))
+        wall, each = skyref.cuboids_timed(x, pick, cands, threads)
+        sample_wall += wall
+        est = float(each.sum()) * len(subs) / k / threads
+        parts[f"level_{L}"] = {"cuboids": len(subs), "sampled": k,
+                               "mean_candidates": int(np.mean([c.size for c in cands])),
+                               "est_s": round(est, 2)}
+        total += est
+    ctx.close()
+    return total, sample_wall, parts
+
+
+# --------------------------------------------------------------------------
 
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -276,20 +389,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="integer_ties")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-sdsc", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-hashcube", action="store_true",
-                    help="single mode: skip the HashCube export timing")
-    ap.add_argument("--verify", action="store_true", help="sha256 of the result vs the golden digest")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
-                    help="single: one skycube per step (the headline); batch: the batched "
-                         "workload; serve: the Poisson request stream")
-    ap.add_argument("--instances", type=int, default=4096, help="batch mode: instance count")
-    ap.add_argument("--requests", type=int, default=600, help="serve mode: request count")
-    ap.add_argument("--qps", type=float, default=0.0, help="serve mode: arrival rate "
-                    "(default 0.7 x the closed-loop capacity)")
-    ap.add_argument("--workers", type=int, default=2, help="serve mode: server workers")
-    ap.add_argument("--verify-every", type=int, default=10,
-                    help="serve mode: check every k-th request against a plain call (0: off)")
+    ap.add_argument("--only-device", action="store_true",
+                    help="only the device-resident skycube loop (for profiling)")
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--qps", type=float, default=0.0)
+    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: every core of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (for scaling HBM counters)")
     args = ap.parse_args()
@@ -314,31 +426,14 @@ def main() -> int:
     import skycube_amd
     from skycube_amd import datagen
 
-    if args.mode == "batch":
-        return bench_batch(args, torch, skycube_amd, datagen)
-    if args.mode == "serve":
-        return bench_serve(args, torch, skycube_amd, datagen)
-
     w = datagen.WORKLOADS[args.workload]
     n, d, max_d = w["n"], w["d"], w["max_d"]
     x = datagen.workload(args.workload)
-    xd = torch.from_numpy(x).to("cuda")
-    torch.cuda.synchronize()
-    ctx = skycube_amd.Context(0)
-    ctx.set_timing(True)
-    ptr = xd.data_ptr()
-    for _ in range(args.warmup):
-        ctx.mdmc_device(ptr, n, d, max_d)
-    torch.cuda.synchronize()
-    stages = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        n_ext = ctx.mdmc_device(ptr, n, d, max_d)
-        stages.append(ctx.stage_times())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    st = {k: round(float(np.mean([s[k] for s in stages])), 3) for k in stages[0]}
+    dev_ms, n_ext, st, ext = bench_device(args, torch, skycube_amd, w, x)
+    if args.only_device:
+        print(json.dumps({"device_ms": round(dev_ms, 3), "stages_ms": st, "n_ext": n_ext}))
+        return 0
+    win = bench_window(args, torch, skycube_amd, datagen, w, x)
     member_ms = st["member"]
     B = member_bytes(n_ext, d, max_d)
     achieved = B / (member_ms * 1e-3) / 1e9
@@ -346,73 +441,50 @@ def main() -> int:
     tpath = os.path.join(ROOT, "profiles", "k_member_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get(args.workload)
-    # the kernel is bound by instruction issue, not bytes: its VALU
-    # instruction count per launch (rocprofv3, profiles/) over the live time
-    valu = None
-    cpath = os.path.join(ROOT, "profiles", "k_member_counters.json")
-    if os.path.exists(cpath):
-        cnt = json.load(open(cpath)).get(args.workload)
-        if cnt:
-            ips = cnt["valu_insts"] / (member_ms * 1e-3)
-            valu = {"bound": "valu", "achieved": round(ips / 1e9, 1), "peak": VALU_PEAK_IPS / 1e9,
-                    "unit": "G wave64 VALU instructions/s", "frac": round(ips / VALU_PEAK_IPS, 3),
-                    "insts_per_launch": cnt["valu_insts"], "source": "profiles/k_member_counters.json"}
     out = {
-        "metric": METRIC, "value": round(ms_step, 3), "unit": "ms/skycube", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "metric": METRIC, "value": round(win["ms"], 3), "unit": "ms/skycube", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(win["ms"], 3),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} n={n} d={d} max_d={max_d} seed={w['seed']}",
                    "cuboids": cuboids_in_scope(d, max_d), "n_ext": n_ext,
-                   "result": "device-resident membership bitmaps (2^d bits per ext point)"},
-        "stages_ms": st,
+                   "window": ("the reference's Execute() window: rows in pinned host memory "
+                              "(copied there untimed, as Init copies them) -> device -> skycube "
+                              "-> HashCube (word, value) -> ids grouped on the device and "
+                              "copied to pinned host memory"),
+                   "hashcube": {"groups": win["groups"], "entries": win["entries"]}},
+        "window_parts_ms": win["parts_ms"],
+        "stages_ms": dict(st, device_total=round(dev_ms, 3)),
         "roofline": {"bound": "hbm", "kernel": f"k_member<d={d}> (code-order groups)",
-                     "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": traffic, "algorithmic_bytes": B},
-        "valu_roofline": valu,
-        # implementation-independent: ext-skyline pairs per second (pruning
-        # can take it past any peak; for comparing implementations only)
-        "ext_pair_rate_per_s": float("%.4g" % (n_ext * n_ext / (ms_step * 1e-3))),
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes": B},
     }
-    # the same skycube through host buffers (sky_mdmc: rows up, ids and
-    # bitmaps down over PCIe into pageable memory), for reference only
-    t0 = time.perf_counter()
-    h_ext, h_dom = ctx.mdmc(x, max_d)
-    out["host_buffers_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
-    out["host_buffers_bytes"] = int(x.nbytes + h_ext.nbytes + h_dom.nbytes)
-    del h_ext, h_dom
-    if not args.no_hashcube:
-        # the reference's timed window (src/main.h:324-336: rows already in
-        # host memory, result materialised as a HashCube): rows up, the
-        # skycube, the HashCube grouped on the device and copied to host
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xd2 = torch.from_numpy(x).to("cuda")
-        torch.cuda.synchronize()
-        ctx.mdmc_device(xd2.data_ptr(), n, d, max_d)
-        t1 = time.perf_counter()
-        gw, gv, go, ids = ctx.mdmc_hashcube()
-        t2 = time.perf_counter()
-        out["reference_window_ms"] = round((t2 - t0) * 1e3, 1)
-        out["hashcube"] = {"ms": round((t2 - t1) * 1e3, 1), "groups": int(gw.size),
-                           "entries": int(ids.size),
-                           "what": "(word, value) -> ids grouped on the device, copied to host"}
-        del xd2, gw, gv, go, ids
-    if args.verify:
-        import hashlib
-        gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
-        ext, dom = ctx.mdmc_fetch(n_ext, d)
-        h = hashlib.sha256(ext.astype("<u4").tobytes() + dom.astype("<u4").tobytes()).hexdigest()
-        out["verified"] = h == gold.get("full", {}).get(args.workload, {}).get("sha256")
+    if not args.no_batch:
+        out["batch"] = bench_batch(args, torch, skycube_amd, datagen)
+    if not args.no_serve:
+        out["serve"] = bench_serve(args, torch, skycube_amd, datagen)
+    if not args.no_sdsc:
+        out["sdsc"] = bench_sdsc(args, torch, skycube_amd, datagen)
+    out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        ext, _ = ctx.mdmc_fetch(n_ext, d) if "ext" not in locals() else (ext, None)
-        cb = cpu_baseline(x, max_d, ext)
-        out["cpu_baseline"] = cb
-        out["speedup_vs_cpu_baseline"] = round(cb["value"] / ms_step, 1)
-    else:
-        out["cpu_baseline"] = None
-    ctx.close()
+        from oracle import skyref
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        if skyref.available():
+            say(f"cpu baseline: reference STSC sample, {threads} threads")
+            t0 = time.perf_counter()
+            total, wall, parts = ref_cpu_baseline(skycube_amd, x, max_d, threads)
+            out["cpu_baseline"] = {
+                "value": round(total * 1e3, 1), "unit": "ms/skycube", "cores": threads,
+                "kind": "reference",
+                "sample": (f"the reference's STSC (MommyQ per cuboid, built from its sources) on "
+                           f"the same input with {threads} threads: the full space in full, "
+                           f"{threads} random cuboids per level with the reference's candidate "
+                           f"lists, each level scaled by its cuboid count; "
+                           f"{wall:.1f} s of sampled work"),
+                "parts": parts, "sample_seconds": round(time.perf_counter() - t0, 1),
+            }
+            out["speedup_vs_cpu_baseline"] = round(total * 1e3 / win["ms"], 1)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -273,13 +273,30 @@ int sky_skyalign(sky_ctx *ctx, const float *rows, uint32_t n_rows, uint32_t d,
  * Results, per cuboid s in 1 .. 2^d - 1, as CSR arrays: sky_off and ext_off
  * (2^d + 1 entries each; entry 0 and skipped cuboids are empty ranges),
  * sky_ids / ext_ids (capacities sky_cap / ext_cap) with ascending row ids per
- * cuboid.  If an id array is too small the call returns SKY_ERR_CAPACITY and
- * the offsets still hold the sizes needed.
+ * cuboid.  If an id array is too small (or NULL) the call returns
+ * SKY_ERR_CAPACITY with the offsets holding the sizes needed; the lattice
+ * stays on the device either way, for sky_sdsc_fetch.
+ *
+ * The walk runs on the device one lattice level at a time (the reference's
+ * level barrier): every cuboid of a level is one instance of a batched
+ * SkyAlign pass over its parent's list, and the lists stay on the device.
  */
 int sky_sdsc(sky_ctx *ctx, const float *rows, uint32_t n, uint32_t d,
              uint32_t max_d, uint64_t *sky_off, uint32_t *sky_ids,
              uint64_t sky_cap, uint64_t *ext_off, uint32_t *ext_ids,
              uint64_t ext_cap);
+
+/* sky_sdsc on rows already in device memory; the lattice stays on the
+ * device for sky_sdsc_fetch. */
+int sky_sdsc_device(sky_ctx *ctx, const float *rows_dev, uint32_t n, uint32_t d,
+                    uint32_t max_d);
+
+/* Copies the lattice of the last sky_sdsc / sky_sdsc_device call into CSR
+ * arrays, as sky_sdsc returns them (SDSC<DIMS>::getLattice,
+ * src/sdsc/SDSC.h:18-54). */
+int sky_sdsc_fetch(sky_ctx *ctx, uint64_t *sky_off, uint32_t *sky_ids,
+                   uint64_t sky_cap, uint64_t *ext_off, uint32_t *ext_ids,
+                   uint64_t ext_cap);
 
 /*
  * Skycube server: continuous batching of independent skycube requests on one

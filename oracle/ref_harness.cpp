@@ -129,13 +129,15 @@ int stsc(const float* rows, uint32_t n, uint32_t max_d, uint32_t threads, uint32
 template <int DIMS>
 double timed_level(const float* rows, const uint32_t* subs, const uint64_t* cand_off,
                    const uint32_t* cand_ids, uint32_t k, uint32_t threads, uint32_t accum,
-                   uint32_t pq) {
+                   uint32_t pq, double* each_s) {
   const double t0 = now_s();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
   for (uint32_t i = 0; i < k; ++i) {
     std::vector<unsigned> sky, ext;
+    const double c0 = now_s();
     cuboid<DIMS>(rows, subs[i], cand_ids + cand_off[i], (uint32_t)(cand_off[i + 1] - cand_off[i]),
                  1, accum, pq, &sky, &ext);
+    if (each_s) each_s[i] = now_s() - c0;
   }
   return now_s() - t0;
 }
@@ -165,35 +167,41 @@ double timed_level(const float* rows, const uint32_t* subs, const uint64_t* cand
 
 extern "C" {
 
+// The lattice of the last ref_stsc call, for ref_stsc_fetch.
+static std::vector<std::vector<unsigned>> g_sky, g_ext;
+
 // The skycube by the reference's STSC walk.  Per cuboid s (1 .. 2^d - 1),
-// CSR lists of sky and ext pids in the order MommyQ leaves them (sky_off /
-// ext_off: 2^d + 1 entries); level_s[k] = seconds of level k (level d is the
-// full space).  With null id arrays (or too small capacities) only the
-// offsets are written and 1 is returned.
+// CSR offsets of the sky and ext pid lists (sky_off / ext_off: 2^d + 1
+// entries); level_s[k] = seconds of level k (level d is the full space).
+// The lists are kept for ref_stsc_fetch.
 int ref_stsc(const float* rows, uint32_t n, uint32_t d, uint32_t max_d, uint32_t threads,
-             uint32_t accum, uint32_t pq, uint64_t* sky_off, uint32_t* sky_ids, uint64_t sky_cap,
-             uint64_t* ext_off, uint32_t* ext_ids, uint64_t ext_cap, double* level_s) {
+             uint32_t accum, uint32_t pq, uint64_t* sky_off, uint64_t* ext_off, double* level_s) {
   if (d < 1 || d > 16 || !sky_off || !ext_off || !level_s) return -1;
   if (max_d == 0 || max_d > d) max_d = d;
-  std::vector<std::vector<unsigned>> sky, ext;
   for (uint32_t k = 0; k <= d; ++k) level_s[k] = 0.0;
-  REF_DISPATCH(d, stsc<D>(rows, n, max_d, threads, accum, pq, sky, ext, level_s));
+  REF_DISPATCH(d, stsc<D>(rows, n, max_d, threads, accum, pq, g_sky, g_ext, level_s));
   const uint32_t S = 1u << d;
   uint64_t os = 0, oe = 0;
   for (uint32_t s = 0; s < S; ++s) {
     sky_off[s] = os;
     ext_off[s] = oe;
-    os += sky[s].size();
-    oe += ext[s].size();
+    os += g_sky[s].size();
+    oe += g_ext[s].size();
   }
   sky_off[S] = os;
   ext_off[S] = oe;
-  if (!sky_ids || !ext_ids || os > sky_cap || oe > ext_cap) return 1;
-  for (uint32_t s = 0; s < S; ++s) {
-    if (!sky[s].empty())
-      std::memcpy(sky_ids + sky_off[s], sky[s].data(), sizeof(uint32_t) * sky[s].size());
-    if (!ext[s].empty())
-      std::memcpy(ext_ids + ext_off[s], ext[s].data(), sizeof(uint32_t) * ext[s].size());
+  return 0;
+}
+
+// The lists of the last ref_stsc call, in the order MommyQ left them, at
+// the offsets it returned.
+int ref_stsc_fetch(uint32_t* sky_ids, uint32_t* ext_ids) {
+  uint64_t os = 0, oe = 0;
+  for (size_t s = 0; s < g_sky.size(); ++s) {
+    if (!g_sky[s].empty()) std::memcpy(sky_ids + os, g_sky[s].data(), sizeof(uint32_t) * g_sky[s].size());
+    if (!g_ext[s].empty()) std::memcpy(ext_ids + oe, g_ext[s].data(), sizeof(uint32_t) * g_ext[s].size());
+    os += g_sky[s].size();
+    oe += g_ext[s].size();
   }
   return 0;
 }
@@ -201,12 +209,14 @@ int ref_stsc(const float* rows, uint32_t n, uint32_t d, uint32_t max_d, uint32_t
 // Wall seconds of k cuboids of the reference's walk run as one parallel
 // level would run them (schedule(dynamic, 1) over `threads`, one thread per
 // cuboid): cuboid i is subspace subs[i] over candidates
-// cand_ids[cand_off[i] .. cand_off[i+1]); results are discarded.
+// cand_ids[cand_off[i] .. cand_off[i+1]); results are discarded.  each_s
+// (nullable, k entries) receives each cuboid's own seconds.
 int ref_cuboids_timed(const float* rows, uint32_t d, const uint32_t* subs, const uint64_t* cand_off,
                       const uint32_t* cand_ids, uint32_t k, uint32_t threads, uint32_t accum,
-                      uint32_t pq, double* wall_s) {
+                      uint32_t pq, double* wall_s, double* each_s) {
   if (d < 1 || d > 16 || !wall_s) return -1;
-  REF_DISPATCH(d, *wall_s = timed_level<D>(rows, subs, cand_off, cand_ids, k, threads, accum, pq));
+  REF_DISPATCH(d, *wall_s = timed_level<D>(rows, subs, cand_off, cand_ids, k, threads, accum, pq,
+                                           each_s));
   return 0;
 }
 

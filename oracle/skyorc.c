@@ -102,13 +102,17 @@ static int has_nan(const float *rows, uint64_t n, int d) {
 
 /* Manhattan norm over the dimensions of s, accumulated in ascending
  * dimension order in FP32, as MommyQ and PQFilter score a tuple
- * (src/hybrid/mommyq.cpp:25-33, src/utilities/pq_filter.h:446-450). */
+ * (src/hybrid/mommyq.cpp:25-33, src/utilities/pq_filter.h:446-450).  It
+ * only orders the candidates (a dominator never has a larger norm).  A row
+ * holding both +inf and -inf over s has a NaN norm and is ordered as -inf:
+ * it has a -inf there, so does every row that dominates it, and their norms
+ * are -inf or NaN too. */
 static inline float score_of(const float *p, uint32_t s, int d) {
   float sum = 0.0f;
   for (int i = 0; i < d; ++i)
     if (s >> i & 1u)
       sum += p[i];
-  return sum;
+  return sum == sum ? sum : -INFINITY;
 }
 
 /* ----------------------------------------------------------------------- */

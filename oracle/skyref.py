@@ -38,11 +38,12 @@ def lib():
     if _lib is None:
         L = ctypes.CDLL(_LIB_PATH)
         L.ref_stsc.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _u32p,
-                               ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64, _f64p]
+                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _u64p,
+                               _f64p]
+        L.ref_stsc_fetch.argtypes = [_u32p, _u32p]
         L.ref_cuboids_timed.argtypes = [_f32p, ctypes.c_uint32, _u32p, _u64p, _u32p,
                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                        ctypes.c_uint32, _f64p]
+                                        ctypes.c_uint32, _f64p, _f64p]
         L.ref_cuboid.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                  _u32p, _u32p, _f64p]
@@ -64,17 +65,13 @@ def stsc(x: np.ndarray, max_d: int | None = None, threads: int = 8):
     so = np.zeros(S + 1, dtype=np.uint64)
     eo = np.zeros(S + 1, dtype=np.uint64)
     lv = np.zeros(d + 1, dtype=np.float64)
-    cap = max(1, n) * 4
-    while True:
-        sky = np.zeros(cap, dtype=np.uint32)
-        ext = np.zeros(cap, dtype=np.uint32)
-        rc = lib().ref_stsc(_p(x, _f32p), n, d, max_d or d, threads, ACCUM, PQ_SIZE, _p(so, _u64p),
-                            _p(sky, _u32p), cap, _p(eo, _u64p), _p(ext, _u32p), cap, _p(lv, _f64p))
-        if rc < 0:
-            raise RuntimeError(f"ref_stsc failed ({rc})")
-        if rc == 0:
-            break
-        cap = int(max(so[S], eo[S])) + 1
+    rc = lib().ref_stsc(_p(x, _f32p), n, d, max_d or d, threads, ACCUM, PQ_SIZE, _p(so, _u64p),
+                        _p(eo, _u64p), _p(lv, _f64p))
+    if rc != 0:
+        raise RuntimeError(f"ref_stsc failed ({rc})")
+    sky = np.zeros(int(so[S]) + 1, dtype=np.uint32)
+    ext = np.zeros(int(eo[S]) + 1, dtype=np.uint32)
+    lib().ref_stsc_fetch(_p(sky, _u32p), _p(ext, _u32p))
     for s in range(1, S):
         sky[so[s]:so[s + 1]].sort()
         ext[eo[s]:eo[s + 1]].sort()
@@ -99,9 +96,9 @@ def cuboid(x: np.ndarray, sub: int, cand=None, threads: int = 1):
     return np.sort(sky[: ns.value]), np.sort(ext[: ne.value]), sec.value
 
 
-def cuboids_timed(x: np.ndarray, subs, cands, threads: int) -> float:
-    """Wall seconds of the cuboids subs[i] over candidates cands[i], run as
-    one parallel level of the reference's walk."""
+def cuboids_timed(x: np.ndarray, subs, cands, threads: int):
+    """(wall seconds, seconds of each cuboid) of the cuboids subs[i] over
+    candidates cands[i], run as one parallel level of the reference's walk."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     d = x.shape[1]
     subs = np.ascontiguousarray(subs, dtype=np.uint32)
@@ -111,8 +108,10 @@ def cuboids_timed(x: np.ndarray, subs, cands, threads: int) -> float:
            else np.zeros(1, np.uint32))
     ids = np.ascontiguousarray(ids, dtype=np.uint32)
     wall = ctypes.c_double()
+    each = np.zeros(max(subs.size, 1), dtype=np.float64)
     rc = lib().ref_cuboids_timed(_p(x, _f32p), d, _p(subs, _u32p), _p(off, _u64p), _p(ids, _u32p),
-                                 subs.size, threads, ACCUM, PQ_SIZE, ctypes.byref(wall))
+                                 subs.size, threads, ACCUM, PQ_SIZE, ctypes.byref(wall),
+                                 _p(each, _f64p))
     if rc != 0:
         raise RuntimeError(f"ref_cuboids_timed failed ({rc})")
-    return wall.value
+    return wall.value, each[: subs.size]

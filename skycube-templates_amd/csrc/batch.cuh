@@ -221,19 +221,20 @@ __global__ void k_bcodes(const float* __restrict__ erows, const uint32_t* __rest
 }
 
 // Survivor pass, one wave64 per survivor j < *count, against the survivors
-// of its own instance (self_scan, shared with the single-instance kernel).
+// of its own instance (self_scan, shared with the single-instance kernel);
+// need_dom also tells dominated (1) from undominated (0) survivors.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_bself(const float* __restrict__ rows,
                                                   const uint32_t* __restrict__ keys,
                                                   const uint32_t* __restrict__ masks,
                                                   const uint32_t* __restrict__ off, uint32_t T,
-                                                  const uint32_t* __restrict__ count,
+                                                  const uint32_t* __restrict__ count, int need_dom,
                                                   uint8_t* __restrict__ status) {
   __shared__ uint32_t qbuf[kBlock / 64][192];
   const uint32_t j = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (j >= *count) return;  // whole wave; no block barrier in this kernel
   const uint32_t t = seg_of(off, T, j);
-  const uint8_t st = self_scan<D>(rows, keys, masks, off[t], off[t + 1], j, 0,
+  const uint8_t st = self_scan<D>(rows, keys, masks, off[t], off[t + 1], j, need_dom,
                                   qbuf[threadIdx.x >> 6]);
   if ((threadIdx.x & 63) == 0) status[j] = st;
 }
@@ -326,14 +327,16 @@ __global__ void k_bidx_first(const uint32_t* __restrict__ brun, const uint32_t* 
 
 // Survivor pass over the mask index, one wave64 per survivor j < *count,
 // against the survivors of its own instance (self_idx, shared with the
-// single-instance kernel; boff is the instance-major bucket table).
+// single-instance kernel; boff is the instance-major bucket table).  With
+// need_dom, dominated (1) and undominated (0) survivors are told apart and
+// minkey is not used.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_bself_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
     const uint2* __restrict__ codes, const uint32_t* __restrict__ off, uint32_t T,
     const uint32_t* __restrict__ count, const uint32_t* __restrict__ boff,
     const uint32_t* __restrict__ sval, const uint32_t* __restrict__ bkey,
-    const uint2* __restrict__ bcode, const uint32_t* __restrict__ minkey,
+    const uint2* __restrict__ bcode, const uint32_t* __restrict__ minkey, int need_dom,
     uint8_t* __restrict__ status) {
   __shared__ uint32_t sLo[kBlock / 64][64];
   __shared__ uint32_t sEx[kBlock / 64][64];
@@ -342,8 +345,41 @@ __global__ __launch_bounds__(kBlock) void k_bself_idx(
   if (j >= *count) return;  // whole wave; no block barrier in this kernel
   const uint32_t t = seg_of(off, T, j);
   const uint8_t st = self_idx<D>(rows, keys, codes, boff + ((size_t)t << D), sval, bkey, bcode, j,
-                                 0, minkey + (size_t)t * 16, sLo[wv], sEx[wv]);
+                                 need_dom, need_dom ? nullptr : minkey + (size_t)t * 16, sLo[wv],
+                                 sEx[wv]);
   if ((threadIdx.x & 63) == 0) status[j] = st;
+}
+
+// SDSC chunk: the rows of candidate i of instance t (cuboid subs[t]),
+// projected onto the D columns of the cuboid, ascending.
+template <int D>
+__global__ void k_sdsc_rows(const float* __restrict__ rows, uint32_t d,
+                            const uint32_t* __restrict__ cand, const uint32_t* __restrict__ subs,
+                            const uint32_t* __restrict__ off, uint32_t T, uint32_t N,
+                            float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t t = seg_of(off, T, i);
+  const float* row = rows + (size_t)cand[i] * d;
+  uint32_t sub = subs[t];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int col = __ffs(sub) - 1;
+    sub &= sub - 1u;
+    out[(size_t)i * D + k] = row[col];
+  }
+}
+
+// SDSC chunk: per candidate position, keep = in sky or ext (not strictly
+// dominated), sky = in the skyline; both cleared up to N first.
+__global__ void k_sdsc_flags(const uint8_t* __restrict__ status, const uint32_t* __restrict__ spid,
+                             const uint32_t* __restrict__ count, uint8_t* __restrict__ keep,
+                             uint8_t* __restrict__ sky) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *count) return;
+  const uint8_t st = status[j];
+  keep[spid[j]] = st != 2;
+  sky[spid[j]] = st == 0;
 }
 
 // Group point ids of the extended skyline as row numbers inside their

@@ -272,6 +272,54 @@ int skyi::run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, ui
   return SKY_OK;
 }
 
+int skyi::batch_buffers(sky_ctx* c, size_t N, size_t T, uint64_t tab) {
+  BatchState& B = c->batch;
+  TRY(ensure(c, B.rows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.key_a, sizeof(uint64_t) * N));
+  TRY(ensure(c, B.key_b, sizeof(uint64_t) * N));
+  TRY(ensure(c, B.rs_key, sizeof(uint64_t) * N));
+  TRY(ensure(c, B.val_a, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.val_b, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.rs_val, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.srows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.skeys, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.spid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.alive, N));
+  TRY(ensure(c, B.s1idx, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.s1rows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.s1keys, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.s1pid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.masks, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.status, N));
+  TRY(ensure(c, B.flag_pt, N));
+  TRY(ensure(c, B.flag_s1, N));
+  TRY(ensure(c, B.eidx, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.erows, sizeof(float) * N * 16));
+  TRY(ensure(c, B.epid, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.pos, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.codes, sizeof(uint2) * N));
+  TRY(ensure(c, B.rank, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.piv, sizeof(uint32_t) * T * 16 * sky::kCodePivots));
+  TRY(ensure(c, B.minkey, sizeof(uint32_t) * T * 16));
+  TRY(ensure(c, B.sminkey, sizeof(uint32_t) * T * 16));
+  TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
+  DevBuf* ib[] = {&B.ikey, &B.ikey2, &B.ival, &B.isort, &B.ibkey, &B.iblocal, &B.ibmk, &B.ibstart,
+                  &B.ibcnt, &B.ibrun};
+  for (DevBuf* b : ib) TRY(ensure(c, *b, sizeof(uint32_t) * N));
+  TRY(ensure(c, B.ibcode, sizeof(uint2) * N));
+  TRY(ensure(c, B.ihead, N));
+  TRY(ensure(c, B.igfirst, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.gcnt, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.goff, sizeof(uint32_t) * (T + 1)));
+  TRY(ensure(c, B.gsum, sizeof(uint2) * (N / 64 + T + 1)));
+  TRY(ensure(c, B.iboff, sizeof(uint32_t) * tab));
+  const size_t tb = prim_tmp_bytes(std::max<uint64_t>({N, T + 1, 1}));
+  TRY(ensure(c, B.tmp, tb));
+  B.tmp_bytes = tb;
+  return SKY_OK;
+}
+
 namespace {
 
 int upload_rows(sky_ctx* c, const float* rows, uint32_t n, uint32_t d) {
@@ -482,53 +530,10 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
     maxT = std::max(maxT, g.T);
     B.groups.push_back(&g);
   }
-  const size_t N = maxN, T = maxT;
-  TRY(ensure(c, B.rows, sizeof(float) * N * 16));
-  TRY(ensure(c, B.key_a, sizeof(uint64_t) * N));
-  TRY(ensure(c, B.key_b, sizeof(uint64_t) * N));
-  TRY(ensure(c, B.rs_key, sizeof(uint64_t) * N));
-  TRY(ensure(c, B.val_a, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.val_b, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.rs_val, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.srows, sizeof(float) * N * 16));
-  TRY(ensure(c, B.skeys, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.spid, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.alive, N));
-  TRY(ensure(c, B.s1idx, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.s1rows, sizeof(float) * N * 16));
-  TRY(ensure(c, B.s1keys, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.s1pid, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.masks, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.status, N));
-  TRY(ensure(c, B.flag_pt, N));
-  TRY(ensure(c, B.flag_s1, N));
-  TRY(ensure(c, B.eidx, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.erows, sizeof(float) * N * 16));
-  TRY(ensure(c, B.epid, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.pos, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.codes, sizeof(uint2) * N));
-  TRY(ensure(c, B.rank, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.piv, sizeof(uint32_t) * T * 16 * sky::kCodePivots));
-  TRY(ensure(c, B.minkey, sizeof(uint32_t) * T * 16));
-  TRY(ensure(c, B.sminkey, sizeof(uint32_t) * T * 16));
-  TRY(ensure(c, B.seg_a, sizeof(uint32_t) * (T + 1)));
-  TRY(ensure(c, B.seg_b, sizeof(uint32_t) * (T + 1)));
-  DevBuf* ib[] = {&B.ikey, &B.ikey2, &B.ival, &B.isort, &B.ibkey, &B.iblocal, &B.ibmk, &B.ibstart,
-                  &B.ibcnt, &B.ibrun};
-  for (DevBuf* b : ib) TRY(ensure(c, *b, sizeof(uint32_t) * N));
-  TRY(ensure(c, B.ibcode, sizeof(uint2) * N));
-  TRY(ensure(c, B.ihead, N));
-  TRY(ensure(c, B.igfirst, sizeof(uint32_t) * (T + 1)));
-  TRY(ensure(c, B.gcnt, sizeof(uint32_t) * (T + 1)));
-  TRY(ensure(c, B.goff, sizeof(uint32_t) * (T + 1)));
-  TRY(ensure(c, B.gsum, sizeof(uint2) * (N / 64 + T + 1)));
   uint64_t tab = 1;
   for (const BatchGroup* g : B.groups)
     if (batch_self_index(g->D, g->T)) tab = std::max<uint64_t>(tab, ((uint64_t)g->T << g->D) + 1);
-  TRY(ensure(c, B.iboff, sizeof(uint32_t) * tab));
-  const size_t tb = prim_tmp_bytes(std::max<uint64_t>({N, T + 1, 1}));
-  TRY(ensure(c, B.tmp, tb));
-  B.tmp_bytes = tb;
+  TRY(batch_buffers(c, maxN, maxT, tab));
   B.sig_rows = rows_dev;
   B.sig_tune = c->tune_ver;
   B.sig_n.assign(n, n + n_inst);
@@ -636,7 +641,12 @@ void free_all(sky_ctx* c) {
   single_bufs(c, bufs);
   for (DevBuf* b : bufs) free_buf(c, *b);
   free_batch_buffers(c);
-  c->has_mdmc = c->has_hc = c->has_batch = false;
+  DevBuf* sd[] = {&c->sd_pool, &c->sd_flag,  &c->sd_lat_off, &c->sd_lat_cnt, &c->sd_lat_sky,
+                  &c->sd_subs, &c->sd_src,   &c->sd_pcnt,    &c->sd_coff,    &c->sd_tasks,
+                  &c->sd_cand, &c->sd_keep,  &c->sd_skyf,    &c->sd_sel,     &c->sd_kcnt,
+                  &c->sd_koff, &c->sd_scnt,  &c->sd_soff,    &c->sd_counters};
+  for (DevBuf* b : sd) free_buf(c, *b);
+  c->has_mdmc = c->has_hc = c->has_batch = c->has_sdsc = false;
 }
 
 }  // namespace
@@ -1057,73 +1067,28 @@ int sky_sdsc(sky_ctx* c, const float* rows, uint32_t n, uint32_t d, uint32_t max
   if (!sky_off || !ext_off || (n && !rows)) return fail(c, SKY_ERR_ARG, "null argument");
   if (max_d == 0 || max_d > d) max_d = d;
   c->has_mdmc = false;
-  const uint32_t S = 1u << d, full = S - 1u;
-  std::vector<std::vector<uint32_t>> lsky(S), lext(S);
+  c->has_hc = false;
   TRY(upload_rows(c, rows, n, d));
-  const float* rows_dev = P<float>(c->rows);
-  // full space over every point (SDSC.cu:274)
-  TRY(skyalign_dev(c, rows_dev, n, d, full, nullptr, n, lsky[full], lext[full]));
-  std::vector<uint32_t> cand;
-  // levels top-down, skipping popcount > max_d (SDSC.cu:291-294)
-  for (int lev = (int)d - 1; lev >= 1; --lev) {
-    if ((uint32_t)lev > max_d) continue;
-    for (uint32_t s = 1; s < full; ++s) {
-      if (__builtin_popcount(s) != lev) continue;
-      // parents in generate_lattice order (descending id); the smallest
-      // sky+ext wins, the full space stands in when parents were skipped
-      // (src/sdsc/SDSC.cu:81-104,365-376)
-      uint32_t src = full;
-      uint32_t first = 0;
-      for (int j = (int)d - 1; j >= 0; --j)
-        if (!(s >> j & 1u)) {
-          first = s | (1u << j);
-          break;
-        }
-      if (!lsky[first].empty()) {
-        size_t best = (size_t)-1;
-        for (int j = (int)d - 1; j >= 0; --j) {
-          if (s >> j & 1u) continue;
-          const uint32_t par = s | (1u << j);
-          const size_t sz = lsky[par].size() + lext[par].size();
-          if (sz < best) {
-            best = sz;
-            src = par;
-          }
-        }
-      }
-      cand.assign(lsky[src].begin(), lsky[src].end());
-      cand.insert(cand.end(), lext[src].begin(), lext[src].end());
-      TRY(ensure(c, c->cand, sizeof(uint32_t) * std::max<size_t>(cand.size(), 1)));
-      if (!cand.empty())
-        HIPCHK(hipMemcpyAsync(c->cand.p, cand.data(), sizeof(uint32_t) * cand.size(),
-                              hipMemcpyHostToDevice, c->stream));
-      TRY(skyalign_dev(c, rows_dev, n, d, s, P<uint32_t>(c->cand), (uint32_t)cand.size(),
-                       lsky[s], lext[s]));
-    }
-  }
-  uint64_t ts = 0, te = 0;
-  for (uint32_t s = 1; s < S; ++s) {
-    ts += lsky[s].size();
-    te += lext[s].size();
-  }
-  uint64_t os = 0, oe = 0;
-  for (uint32_t s = 0; s < S; ++s) {
-    sky_off[s] = os;
-    ext_off[s] = oe;
-    if (s == 0) continue;
-    if (sky_ids && os + lsky[s].size() <= sky_cap)
-      std::copy(lsky[s].begin(), lsky[s].end(), sky_ids + os);
-    if (ext_ids && oe + lext[s].size() <= ext_cap)
-      std::copy(lext[s].begin(), lext[s].end(), ext_ids + oe);
-    os += lsky[s].size();
-    oe += lext[s].size();
-  }
-  sky_off[S] = os;
-  ext_off[S] = oe;
-  if (!sky_ids || !ext_ids || ts > sky_cap || te > ext_cap)
-    return fail(c, SKY_ERR_CAPACITY, "sdsc needs %llu sky ids and %llu ext ids",
-                (unsigned long long)ts, (unsigned long long)te);
-  return SKY_OK;
+  TRY(sdsc_walk(c, P<float>(c->rows), n, d, max_d));
+  return sdsc_copy_out(c, sky_off, sky_ids, sky_cap, ext_off, ext_ids, ext_cap);
+}
+
+int sky_sdsc_device(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d) {
+  TRY(check_ctx(c));
+  TRY(check_dims(c, d));
+  if (n && !rows_dev) return fail(c, SKY_ERR_ARG, "null argument");
+  if (max_d == 0 || max_d > d) max_d = d;
+  c->has_mdmc = false;
+  c->has_hc = false;
+  return sdsc_walk(c, rows_dev, n, d, max_d);
+}
+
+int sky_sdsc_fetch(sky_ctx* c, uint64_t* sky_off, uint32_t* sky_ids, uint64_t sky_cap,
+                   uint64_t* ext_off, uint32_t* ext_ids, uint64_t ext_cap) {
+  TRY(check_ctx(c));
+  if (!sky_off || !ext_off) return fail(c, SKY_ERR_ARG, "null argument");
+  if (!c->has_sdsc) return fail(c, SKY_ERR_STATE, "no SDSC result is held");
+  return sdsc_copy_out(c, sky_off, sky_ids, sky_cap, ext_off, ext_ids, ext_cap);
 }
 
 }  // extern "C"

@@ -112,6 +112,15 @@ struct sky_ctx {
   uint32_t res_n_ext = 0, res_d = 0, res_W = 0;
   // set during sky_mdmc_device_noalloc: buffers come from the scratch array
   skyi::Carve* carve = nullptr;
+  // the last SDSC lattice (sdsc.hip): per cuboid s an ascending id list (sky
+  // and ext together) at sd_pool[sd_lat_off[s] ..] of sd_lat_cnt[s] ids,
+  // sd_flag 1 for sky; sd_lat_sky[s] skyline sizes; index 0 = all points
+  skyi::DevBuf sd_pool, sd_flag, sd_lat_off, sd_lat_cnt, sd_lat_sky, sd_subs, sd_src, sd_pcnt,
+      sd_coff, sd_tasks, sd_cand, sd_keep, sd_skyf, sd_sel, sd_kcnt, sd_koff, sd_scnt, sd_soff,
+      sd_counters;
+  bool has_sdsc = false;
+  uint64_t sd_used = 0;
+  uint32_t sd_d = 0;
 };
 
 namespace skyi {
@@ -225,6 +234,16 @@ int seg_count_scan(sky_ctx* c, const uint8_t* flags, const uint32_t* seg, uint32
 int select_dev(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t* count);
 int exclusive_scan_dev(sky_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n);
 
+// The SDSC lattice of n x d rows in device memory, kept on the context
+// (sdsc.hip), and its copy into CSR arrays in cuboid order.
+int sdsc_walk(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, uint32_t max_d);
+int sdsc_copy_out(sky_ctx* c, uint64_t* sky_off, uint32_t* sky_ids, uint64_t sky_cap,
+                  uint64_t* ext_off, uint32_t* ext_ids, uint64_t ext_cap);
+
+// The shared buffers of the batched path for groups of up to N points and T
+// instances, with a median-mask bucket table of `tab` entries (core.hip).
+int batch_buffers(sky_ctx* c, size_t N, size_t T, uint64_t tab);
+
 // ---- stages templated on the dimensionality (stage_single.hip, stage_batch.hip)
 
 // Extended-skyline split of a candidate set over a subspace of D columns:
@@ -242,6 +261,25 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey);
 // One batch group's whole pass, enqueued on the context stream.
 template <int D>
 int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev);
+
+// One chunk of an SDSC lattice level: T cuboids of D columns (subs[t]), each
+// over its candidates cand[off[t] .. off[t+1]) (row ids of the n x d rows,
+// ascending per cuboid), N candidates in all, prefilter tasks of 256
+// positions; leaves keep[i] (sky or ext) and sky[i] per candidate position.
+struct SdscChunk {
+  const float* rows = nullptr;
+  uint32_t d = 0;
+  const uint32_t* subs = nullptr;
+  const uint32_t* cand = nullptr;
+  const uint32_t* off = nullptr;
+  const uint2* tasks = nullptr;
+  uint32_t ntasks = 0, T = 0, N = 0;
+  uint8_t* keep = nullptr;
+  uint8_t* sky = nullptr;
+  uint32_t* cnt = nullptr;  // kGroupCounters device words ([0] bit 1: NaN seen)
+};
+template <int D>
+int sdsc_chunk_d(sky_ctx* c, const SdscChunk& k);
 
 }  // namespace skyi
 

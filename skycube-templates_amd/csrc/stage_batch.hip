@@ -134,7 +134,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
     hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
                        P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes), s1off, T,
                        cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort), P<uint32_t>(B.ibkey),
-                       P<uint2>(B.ibcode), P<uint32_t>(B.sminkey), P<uint8_t>(B.status));
+                       P<uint2>(B.ibcode), P<uint32_t>(B.sminkey), 0, P<uint8_t>(B.status));
     LAUNCHCHK();
   } else {
     hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream,
@@ -143,7 +143,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
     LAUNCHCHK();
     hipLaunchKernelGGL(sky::k_bself<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
                        P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint32_t>(B.masks), s1off, T,
-                       cnt + 1, P<uint8_t>(B.status));
+                       cnt + 1, 0, P<uint8_t>(B.status));
     LAUNCHCHK();
   }
   hipLaunchKernelGGL(sky::k_bext_flags, dim3(gN), dim3(kBlock), 0, c->stream,
@@ -220,6 +220,82 @@ int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   return batch_member<D, sky::member_points_alt<D>()>(c, g, eoff, tu);
 }
 
+// One chunk of an SDSC lattice level: the T cuboids of D columns in k, each
+// a batched instance over its candidates, through the batched split with
+// dominance told apart; leaves keep / sky flags per candidate position.
+template <int D>
+int sdsc_chunk_d(sky_ctx* c, const SdscChunk& k) {
+  BatchState& B = c->batch;
+  const uint32_t N = k.N, T = k.T;
+  uint32_t* cnt = k.cnt;  // [0] err [1] survivors
+  const uint32_t* off = k.off;
+  const unsigned gN = grid_for(N);
+  TRY(fill32(c, cnt, kGroupCounters, 0));
+  TRY(fill8(c, k.keep, N, 0));
+  TRY(fill8(c, k.sky, N, 0));
+  hipLaunchKernelGGL(sky::k_sdsc_rows<D>, dim3(gN), dim3(kBlock), 0, c->stream, k.rows, k.d,
+                     k.cand, k.subs, off, T, N, P<float>(B.rows));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bscore<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.rows), off,
+                     T, N, P<uint64_t>(B.key_a), P<uint32_t>(B.val_a), cnt);
+  LAUNCHCHK();
+  HIPCHK(prim_sort_pairs(P<const uint64_t>(B.key_a), P<uint64_t>(B.key_b), P<const uint32_t>(B.val_a),
+                         P<uint32_t>(B.val_b), P<uint64_t>(B.rs_key), P<uint32_t>(B.rs_val), N,
+                         32 + tbits_of(T), B.tmp.p, c->stream));
+  hipLaunchKernelGGL(sky::k_bgather<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.rows),
+                     P<uint64_t>(B.key_b), P<uint32_t>(B.val_b), N, P<float>(B.srows),
+                     P<uint32_t>(B.skeys), P<uint32_t>(B.spid));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bprune<D>, dim3(k.ntasks), dim3(kBlock), 0, c->stream,
+                     P<float>(B.srows), P<uint32_t>(B.skeys), off, k.tasks, c->tune[D].pruners,
+                     P<uint8_t>(B.alive));
+  LAUNCHCHK();
+  TRY(select_dev(c, P<uint8_t>(B.alive), N, P<uint32_t>(B.s1idx), cnt + 1));
+  uint32_t* s1off = P<uint32_t>(B.seg_b);
+  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off));
+  hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<float>(B.srows), P<uint32_t>(B.skeys), P<uint32_t>(B.spid),
+                     P<uint32_t>(B.s1idx), cnt + 1, P<float>(B.s1rows), P<uint32_t>(B.s1keys),
+                     P<uint32_t>(B.s1pid));
+  LAUNCHCHK();
+  hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
+                     P<float>(B.s1rows), s1off, T, P<uint32_t>(B.piv));
+  LAUNCHCHK();
+  if (batch_self_index(D, T)) {
+    hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), s1off, T, cnt + 1, P<uint32_t>(B.piv),
+                       P<uint2>(B.codes));
+    LAUNCHCHK();
+    TRY(batch_mask_sort<D>(c, s1off, T, N, cnt + 1));
+    hipLaunchKernelGGL(sky::k_bidx_bounds, dim3(grid_for(((uint64_t)T << D) + 1)), dim3(kBlock), 0,
+                       c->stream, P<uint32_t>(B.ikey2), cnt + 1, T << D, P<uint32_t>(B.iboff));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bidx_gather, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<uint32_t>(B.isort), P<uint32_t>(B.s1keys), P<uint2>(B.codes), cnt + 1,
+                       nullptr, T, P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes), s1off, T,
+                       cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort), P<uint32_t>(B.ibkey),
+                       P<uint2>(B.ibcode), nullptr, 1, P<uint8_t>(B.status));
+    LAUNCHCHK();
+  } else {
+    hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), s1off, T, cnt + 1, P<uint32_t>(B.piv),
+                       P<uint32_t>(B.masks));
+    LAUNCHCHK();
+    hipLaunchKernelGGL(sky::k_bself<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
+                       P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint32_t>(B.masks), s1off, T,
+                       cnt + 1, 1, P<uint8_t>(B.status));
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(sky::k_sdsc_flags, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint8_t>(B.status), P<uint32_t>(B.s1pid), cnt + 1, k.keep, k.sky);
+  LAUNCHCHK();
+  return SKY_OK;
+}
+
 template int batch_group_d<SKY_D>(sky_ctx*, BatchGroup&, const float*);
+template int sdsc_chunk_d<SKY_D>(sky_ctx*, const SdscChunk&);
 
 }  // namespace skyi

@@ -47,7 +47,7 @@ EXPORTS = (
     "sky_batch_fetch_into", "sky_ctx_get_option", "sky_batch_autotune",
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
-    "sky_mdmc_device_noalloc",
+    "sky_mdmc_device_noalloc", "sky_sdsc_device", "sky_sdsc_fetch",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -102,6 +102,10 @@ def load():
     L.sky_sdsc.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                            ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p,
                            ctypes.c_uint64]
+    L.sky_sdsc_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_uint32]
+    L.sky_sdsc_fetch.argtypes = [ctypes.c_void_p, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p,
+                                 ctypes.c_uint64]
     L.sky_mdmc_hashcube.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u32p, _u32p, _u64p, _u32p,
                                      ctypes.c_uint64, ctypes.c_uint64]
     L.sky_batch_mdmc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
@@ -302,6 +306,39 @@ class Context:
         self._check(self._L.sky_mdmc_fetch(self._h, None, _ptr(dom, _u32p), dom.size))
         return ext[:k].copy(), dom[: k * W].reshape(k, W)
 
+    def mdmc_raw(self, rows_ptr: int, n: int, d: int, max_d: int | None, ext_ptr: int) -> int:
+        """sky_mdmc on raw host pointers (rows, and room for n ext ids): the
+        bitmaps stay on the device.  Returns n_ext."""
+        ne = np.zeros(1, dtype=np.uint32)
+        self._check(self._L.sky_mdmc(self._h, ctypes.cast(ctypes.c_void_p(rows_ptr), _f32p), n, d,
+                                     0 if max_d is None else max_d, ctypes.cast(ctypes.c_void_p(ext_ptr), _u32p),
+                                     _ptr(ne, _u32p), None, 0))
+        return int(ne[0])
+
+    def hashcube_sizes(self):
+        """(groups, entries) of the HashCube of the last MDMC result (built
+        on the device if not built yet)."""
+        ng = np.zeros(1, dtype=np.uint64)
+        nt = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.sky_mdmc_hashcube(self._h, _ptr(ng, _u64p), _ptr(nt, _u64p), None,
+                                              None, None, None, 0, 0))
+        return int(ng[0]), int(nt[0])
+
+    def hashcube_into(self, gw_ptr: int, gv_ptr: int, go_ptr: int, ids_ptr: int,
+                      cap_groups: int, cap_entries: int):
+        """The HashCube of the last MDMC result into caller buffers given by
+        raw host pointers (uint32 words, uint32 values, uint64 offsets of
+        cap_groups + 1, uint32 ids); returns (groups, entries)."""
+        ng = np.zeros(1, dtype=np.uint64)
+        nt = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.sky_mdmc_hashcube(self._h, _ptr(ng, _u64p), _ptr(nt, _u64p),
+                                              ctypes.cast(ctypes.c_void_p(gw_ptr), _u32p),
+                                              ctypes.cast(ctypes.c_void_p(gv_ptr), _u32p),
+                                              ctypes.cast(ctypes.c_void_p(go_ptr), _u64p),
+                                              ctypes.cast(ctypes.c_void_p(ids_ptr), _u32p), cap_groups,
+                                              cap_entries))
+        return int(ng[0]), int(nt[0])
+
     def mdmc_device(self, rows_dev_ptr: int, n: int, d: int, max_d: int | None = None) -> int:
         """Runs on rows already in device memory; results stay on the device."""
         ne = np.zeros(1, dtype=np.uint32)
@@ -425,11 +462,26 @@ class Context:
                               _ptr(so, _u64p), None, 0, _ptr(eo, _u64p), None, 0)
         if rc != -5:
             self._check(rc)
+        return self.sdsc_fetch(d)
+
+    def sdsc_device(self, rows_dev_ptr: int, n: int, d: int, max_d: int | None = None) -> None:
+        """The SDSC lattice of rows already in device memory, kept on the
+        device (sdsc_fetch copies it out)."""
+        self._check(self._L.sky_sdsc_device(self._h, ctypes.c_void_p(rows_dev_ptr), n, d,
+                                            0 if max_d is None else max_d))
+
+    def sdsc_fetch(self, d: int):
+        """CSR lattice of the last SDSC call: (sky_off, sky_ids, ext_off, ext_ids)."""
+        S = 1 << d
+        so = np.zeros(S + 1, dtype=np.uint64)
+        eo = np.zeros(S + 1, dtype=np.uint64)
+        rc = self._L.sky_sdsc_fetch(self._h, _ptr(so, _u64p), None, 0, _ptr(eo, _u64p), None, 0)
+        if rc != -5:
+            self._check(rc)
         sky = np.zeros(max(int(so[S]), 1), dtype=np.uint32)
         ext = np.zeros(max(int(eo[S]), 1), dtype=np.uint32)
-        self._check(self._L.sky_sdsc(self._h, _ptr(x, _f32p), n, d, 0 if max_d is None else max_d,
-                                     _ptr(so, _u64p), _ptr(sky, _u32p), sky.size, _ptr(eo, _u64p),
-                                     _ptr(ext, _u32p), ext.size))
+        self._check(self._L.sky_sdsc_fetch(self._h, _ptr(so, _u64p), _ptr(sky, _u32p), sky.size,
+                                           _ptr(eo, _u64p), _ptr(ext, _u32p), ext.size))
         return so, sky[: int(so[S])].copy(), eo, ext[: int(eo[S])].copy()
 
 

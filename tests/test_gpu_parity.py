@@ -221,3 +221,89 @@ def test_mdmc_independent_16d_sampled(ctx):
         assert not np.all(x < x[i], axis=1).any(), i
     for i in rng.choice(np.flatnonzero(~inside), size=16, replace=False):
         assert np.all(x < x[i], axis=1).any(), i
+
+
+def test_infinities(ctx):
+    """Rows with +inf and -inf, some holding both (a NaN Manhattan norm): a
+    weak dominator such as (-inf, +inf) of p = (0, +inf) must be found in
+    the skyalign and SDSC paths too, whose survivor pass cuts large buckets
+    at the point's norm."""
+    rng = np.random.default_rng(77)
+    x = rng.integers(0, 6, size=(2000, 2)).astype(np.float32)
+    x[:40, 1] = np.inf
+    x[40] = [-np.inf, np.inf]
+    x[41] = [0.0, np.inf]
+    x[42] = [np.inf, -np.inf]
+    sky, ext = ctx.skyalign(x, 0b11)
+    st = skyorc.skyalign(x, 0b11)
+    assert np.array_equal(sky, np.flatnonzero(st == 0))
+    assert np.array_equal(ext, np.flatnonzero(st == 1))
+    assert 41 not in sky.tolist()
+    check_mdmc(ctx, x, tag="inf 2d")
+    y = rng.integers(0, 4, size=(900, 7)).astype(np.float32)
+    y[rng.random(y.shape) < 0.04] = np.inf
+    y[rng.random(y.shape) < 0.04] = -np.inf
+    check_mdmc(ctx, y, tag="inf 7d")
+    check_mdmc(ctx, y, 3, tag="inf 7d cap")
+    so, sk, eo, ex = ctx.sdsc(y[:300], 4)
+    dom, sdom, _, _ = skyorc.stsc(y[:300], 4)
+    D = npref.unpack_bits(dom, 128)
+    SD = npref.unpack_bits(sdom, 128)
+    for s in range(1, 128):
+        if bin(s).count("1") > 4 and s != 127:
+            continue
+        assert np.array_equal(sk[so[s]:so[s + 1]], np.flatnonzero(~D[:, s])), s
+        assert np.array_equal(ex[eo[s]:eo[s + 1]], np.flatnonzero(D[:, s] & ~SD[:, s])), s
+
+
+@pytest.mark.parametrize("n,d,max_d", [(3000, 12, 12), (2500, 13, 6), (2000, 14, 5),
+                                       (1500, 16, 4), (1200, 15, 15)])
+def test_sdsc_wide_d_matches_stsc(ctx, n, d, max_d):
+    """The device lattice walk at d = 12..16, with and without -m caps,
+    against the oracle's STSC walk: every cuboid's sky and ext lists."""
+    rng = np.random.default_rng(7000 + d + max_d)
+    x = datagen.anticorrelated(n, d, 70 + d) if d == 12 else \
+        rng.integers(0, 5, size=(n, d)).astype(np.float32)
+    _say(f"sdsc n={n} d={d} max_d={max_d}")
+    so, sky, eo, ext = ctx.sdsc(x, max_d)
+    dom, sdom, _, _ = skyorc.stsc(x, max_d)
+    S = 1 << d
+    D = npref.unpack_bits(dom, S)
+    SD = npref.unpack_bits(sdom, S)
+    for s in range(1, S):
+        if bin(s).count("1") > max_d and s != S - 1:
+            assert so[s + 1] == so[s] and eo[s + 1] == eo[s], s
+            continue
+        assert np.array_equal(sky[so[s]:so[s + 1]], np.flatnonzero(~D[:, s])), s
+        assert np.array_equal(ext[eo[s]:eo[s + 1]], np.flatnonzero(D[:, s] & ~SD[:, s])), s
+
+
+def test_sdsc_anticorrelated_full_is_golden(ctx):
+    """The full anticorrelated workload (n = 500,000, d = 12, 4,095 cuboids)
+    through the device SDSC walk: the skyline sizes hash to the golden value
+    and the skylines, turned into per-point bitmaps, hash to the golden
+    digest of the oracle's MDMC result."""
+    import hashlib
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "golden.json")))["full"]["anticorrelated_plane"]
+    x = datagen.workload("anticorrelated_plane")
+    _say("sdsc anticorrelated_plane n=500000 d=12")
+    so, sky, eo, ext = ctx.sdsc(x, 12)
+    S = 1 << 12
+    sizes = np.zeros(S, dtype=np.int64)
+    sizes[1:] = np.diff(so.astype(np.int64))[1:S]
+    assert int(sizes[1:].sum()) == gold["sky_size_total"]
+    assert hashlib.sha256(sizes.astype("<i8").tobytes()).hexdigest() == gold["sky_sizes_sha256"]
+    full = S - 1
+    pts = np.sort(np.concatenate([sky[so[full]:so[full + 1]], ext[eo[full]:eo[full + 1]]]))
+    assert pts.size == gold["n_ext"]
+    pos = np.full(x.shape[0], -1, dtype=np.int64)
+    pos[pts] = np.arange(pts.size)
+    dom = np.full((pts.size, S // 32), 0xFFFFFFFF, dtype=np.uint32)
+    for s in range(1, S):
+        ids = sky[so[s]:so[s + 1]]
+        dom[pos[ids], s >> 5] &= np.uint32(~(1 << (s & 31)) & 0xFFFFFFFF)
+    h = hashlib.sha256(pts.astype("<u4").tobytes() + dom.astype("<u4").tobytes()).hexdigest()
+    assert h == gold["sha256"]

@@ -206,3 +206,29 @@ def test_nan_rejected():
     x[2, 1] = np.nan
     with pytest.raises(RuntimeError):
         skyorc.mdmc(x)
+
+
+@pytest.mark.parametrize("d", [2, 5, 7, 11])
+def test_infinities_match_numpy(d):
+    """Rows with +inf and -inf, some holding both (a NaN Manhattan norm,
+    which the oracle orders as -inf): every algorithm of the oracle against
+    the numpy statement."""
+    rng = np.random.default_rng(60 + d)
+    n = 60 if d <= 7 else 30
+    x = rng.integers(0, 4, size=(n, d)).astype(np.float32)
+    x[rng.random(x.shape) < 0.08] = np.inf
+    x[rng.random(x.shape) < 0.08] = -np.inf
+    x[0, :2] = [-np.inf, np.inf]
+    S = 1 << d
+    rdom, rsdom = npref.brute_rows(x)
+    dom, sdom = skyorc.brute(x)
+    assert np.array_equal(npref.unpack_bits(dom, S)[:, 1:], rdom[:, 1:])
+    assert np.array_equal(npref.unpack_bits(sdom, S)[:, 1:], rsdom[:, 1:])
+    st_dom, st_sdom, _, _ = skyorc.stsc(x, d)
+    assert np.array_equal(npref.unpack_bits(st_dom, S)[:, 1:], rdom[:, 1:])
+    ext, mdom = skyorc.mdmc(x, d)
+    assert np.array_equal(ext, np.flatnonzero(~rsdom[:, S - 1]))
+    assert np.array_equal(npref.unpack_bits(mdom, S)[:, 1:], rdom[ext][:, 1:])
+    for sub in (1, S - 1, (S - 1) & 0b10110):
+        st = skyorc.skyalign(x, sub)
+        assert np.array_equal(st, np.where(rsdom[:, sub], 2, np.where(rdom[:, sub], 1, 0)))
