@@ -475,15 +475,32 @@ __device__ __forceinline__ uint32_t low_closure(uint32_t x, int D) {
 // Bits of word k whose subspace is in scope: s != 0, s < 2^D, popcount(s) <=
 // max_d.
 __device__ __forceinline__ uint32_t scope_mask(uint32_t k, int D, int max_d) {
+  // bits b of a word with popcount(b) <= r, for r = 0..4 (r >= 5: all)
   const int rem = max_d - __popc(k);
-  uint32_t m = 0;
-  if (rem >= 0) {
-#pragma unroll
-    for (int b = 0; b < 32; ++b) m |= (uint32_t)(__popc(b) <= rem) << b;
-  }
+  uint32_t m = rem >= 5 ? 0xFFFFFFFFu : 0x7FFFFFFFu;
+  m = rem <= 3 ? 0x177F7FFFu : m;
+  m = rem <= 2 ? 0x0117177Fu : m;
+  m = rem <= 1 ? 0x00010117u : m;
+  m = rem <= 0 ? 0x00000001u : m;
+  m = rem < 0 ? 0u : m;
   if (D < 5) m &= (1u << (1 << D)) - 1u;
   if (k == 0) m &= ~1u;
   return m;
+}
+
+// Bits b of a word (the subsets b of the 5 low columns) with b inside zl,
+// and with exactly one column of b outside zl (zl: 5-bit column mask).
+__device__ __forceinline__ void low_subsets(uint32_t zl, uint32_t* in, uint32_t* one_out) {
+  uint32_t m0 = 1u;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    if ((zl >> i) & 1u) m0 |= m0 << (1u << i);
+  uint32_t m1 = 0u;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    if (!((zl >> i) & 1u)) m1 |= m0 << (1u << i);
+  *in = m0;
+  *one_out = m1;
 }
 
 // Column minima of the extended skyline, as order keys, for the boundary
@@ -1140,12 +1157,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     const uint32_t rg = s_rgt[pp];
     const uint32_t lowR = ~z & 31u;
     uint32_t m0 = 0, m1 = 0;  // bits b whose low columns outside Z number 0 / 1
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const int c = __popc((uint32_t)b & lowR);
-      m0 |= (uint32_t)(c == 0) << b;
-      m1 |= (uint32_t)(c == 1) << b;
-    }
+    low_subsets(~lowR & 31u, &m0, &m1);
     for (int k = tid; k < W; k += NT) {
       const int ph = __popc(((uint32_t)k << 5) & ~z);
       const uint32_t r0 = ph == 0 ? m0 : 0u;
