@@ -273,6 +273,25 @@ def bench_serve(args, torch, skycube_amd, datagen):
             break
         qps = qps * min(4.0, 0.7 / max(busy, 0.05))
     srv.close()
+    # the same stream at the same offered rate on a server with another
+    # worker count (fewer workers: larger batches, longer waits)
+    compare = None
+    if args.compare_workers and args.compare_workers != args.workers and not errors:
+        alt = skycube_amd.Server(0, hbm_budget=0, workers=args.compare_workers)
+        for i in [alt.submit(m) for m in mats]:  # warm-up pass
+            alt.result(i)
+        st0 = alt.stats()
+        lat2, bsz2, _, t2, err2 = _serve_stream(alt, mats, stream, 0)
+        st1 = alt.stats()
+        alt.close()
+        good2 = [v for v in lat2 if v is not None]
+        say(f"serve: {args.compare_workers} workers at {qps:.0f}/s")
+        compare = {"workers": args.compare_workers, "offered_qps": round(qps, 1),
+                   "latency_ms": {"p50": _pct(good2, 50), "p99": _pct(good2, 99),
+                                  "max": _pct(good2, 100)},
+                   "batch_size": {"mean": round(float(np.mean(bsz2)), 2), "max": int(max(bsz2))},
+                   "server_busy_frac": round((st1["busy_ms"] - st0["busy_ms"]) / (t2 * 1e3), 3),
+                   "errors": err2 or None}
     ok = 0
     with skycube_amd.Context(0) as ctx:
         for k, (e0, d0) in kept.items():
@@ -291,6 +310,7 @@ def bench_serve(args, torch, skycube_amd, datagen):
         "batch_size": {"mean": round(float(np.mean(bsz)), 2), "max": int(max(bsz))},
         "verified_alone": f"{ok}/{len(kept)} sampled requests identical to the same rows alone",
         "errors": errors or None,
+        "same_load_other_workers": compare,
     }
 
 
@@ -399,7 +419,10 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--qps", type=float, default=0.0)
-    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--compare-workers", type=int, default=2,
+                    help="also run the serving stream at the same rate with this many workers "
+                         "(0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (0: every core of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -218,6 +218,22 @@ int sky_batch_mdmc(sky_ctx *ctx, const float *rows, int rows_on_device,
                    uint32_t n_inst, const uint32_t *n, const uint32_t *d,
                    const uint32_t *max_d, int use_graph, uint64_t *total_ext);
 
+/* sky_batch_mdmc over instances in separate device allocations:
+ * inst_rows_dev[t] holds instance t's n[t] x d[t] FP32 rows in device
+ * memory.  The pass is enqueued on the context's stream (sky_ctx_stream)
+ * behind the work already there, so a caller that uploads rows on another
+ * stream makes that stream's work visible first (hipStreamWaitEvent on the
+ * context's stream).  No graph; descriptors go up asynchronously from
+ * pinned staging, and the only host wait is the one that reads the result
+ * counts.  Results as for sky_batch_mdmc.  No reference counterpart. */
+int sky_batch_mdmc_dev(sky_ctx *ctx, const float *const *inst_rows_dev,
+                       uint32_t n_inst, const uint32_t *n, const uint32_t *d,
+                       const uint32_t *max_d, uint64_t *total_ext);
+
+/* The HIP stream (a hipStream_t) that every call on the context enqueues
+ * its work on, for callers that order their own work with it. */
+void *sky_ctx_stream(sky_ctx *ctx);
+
 /* Results of the last batch, in instance order: ext_off and dom_off
  * (n_inst + 1 entries each) delimit instance t's extended-skyline row ids
  * (local, ascending) in ext_pids and its bitmaps (sky_words_per_point(d[t])
@@ -305,14 +321,18 @@ int sky_sdsc_fetch(sky_ctx *ctx, uint64_t *sky_off, uint32_t *sky_ids,
  *
  * sky_server_submit queues a request (the rows are copied; NaN input and a
  * request whose device footprint exceeds one worker's share of the budget are
- * rejected at once with SKY_ERR_NAN / SKY_ERR_OOM_BUDGET).  `workers` threads,
- * each with its own context and stream, repeatedly take every queued request,
- * first come first served, up to `max_batch` requests and hbm_budget_bytes /
- * workers of estimated device memory, and compute them as one batched MDMC
- * pass (sky_batch_mdmc), so that requests arriving while the device is busy
- * join the next batch and one worker's copies overlap another's kernels.
- * Every request's result is identical to sky_mdmc on the same rows alone.
- * hbm_budget_bytes = 0 means 80% of the device's free memory.
+ * rejected at once with SKY_ERR_NAN / SKY_ERR_OOM_BUDGET) and enqueues the
+ * upload of its rows at once, so that they wait resident in device memory
+ * (a stream-ordered pool holding up to 1/16 of the budget; past that, the
+ * worker uploads them when it takes the request).  `workers` threads, each
+ * with its own context and stream, repeatedly take every queued request,
+ * first come first served, up to `max_batch` requests and an equal share of
+ * the rest of the budget of estimated device memory, and compute them as
+ * one batched MDMC pass over the resident rows (sky_batch_mdmc_dev), so that
+ * requests arriving while the device is busy join the next batch and one
+ * worker's host work overlaps another's kernels.  Every request's result is
+ * identical to sky_mdmc on the same rows alone.  hbm_budget_bytes = 0 means
+ * 80% of the device's free memory.
  */
 typedef struct sky_server sky_server;
 

@@ -42,15 +42,15 @@ __global__ void k_fill_u8(uint8_t* __restrict__ p, uint64_t n, uint8_t v) {
 }
 
 // The group's rows, contiguous: point i of the group is row i - off[t] of
-// instance t, found at float offset src_foff[t] of the caller's array.
+// instance t, whose rows start at device address src[t] (instances need not
+// share an allocation).
 template <int D>
-__global__ void k_bcopy(const float* __restrict__ src, const uint64_t* __restrict__ src_foff,
-                        const uint32_t* __restrict__ off, uint32_t T, uint32_t N,
-                        float* __restrict__ dst) {
+__global__ void k_bcopy(const uint64_t* __restrict__ src, const uint32_t* __restrict__ off,
+                        uint32_t T, uint32_t N, float* __restrict__ dst) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const uint32_t t = seg_of(off, T, i);
-  const float* row = src + src_foff[t] + (size_t)(i - off[t]) * D;
+  const float* row = reinterpret_cast<const float*>(src[t]) + (size_t)(i - off[t]) * D;
 #pragma unroll
   for (int k = 0; k < D; ++k) dst[(size_t)i * D + k] = row[k];
 }

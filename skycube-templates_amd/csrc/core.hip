@@ -460,10 +460,18 @@ void free_batch_buffers(sky_ctx* c) {
   free_batch(c);
   BatchState& B = c->batch;
   for (BatchGroup& g : B.pool) {
-    DevBuf* gb[] = {&g.d_off, &g.d_src_foff, &g.d_tasks, &g.d_maxd, &g.ext_pids, &g.eoff, &g.dom,
+    DevBuf* gb[] = {&g.d_off, &g.d_src, &g.d_tasks, &g.d_maxd, &g.ext_pids, &g.eoff, &g.dom,
                     &g.cnt};
     for (DevBuf* b : gb) free_buf(c, *b);
   }
+  if (B.desc_done) {
+    (void)hipEventSynchronize(B.desc_done);
+    (void)hipEventDestroy(B.desc_done);
+    B.desc_done = nullptr;
+  }
+  if (B.desc_host) (void)hipHostFree(B.desc_host);
+  B.desc_host = nullptr;
+  B.desc_cap = 0;
   DevBuf* sb[] = {&B.rows,  &B.key_a,  &B.key_b,  &B.val_a, &B.val_b,  &B.srows,  &B.skeys,
                   &B.spid,  &B.alive,  &B.s1idx,  &B.s1rows, &B.s1keys, &B.s1pid, &B.masks,
                   &B.status, &B.flag_pt, &B.flag_s1, &B.eidx, &B.erows, &B.epid,  &B.pos,
@@ -476,26 +484,60 @@ void free_batch_buffers(sky_ctx* c) {
 }
 
 // Builds the groups of a batch and every device buffer it needs.
-int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32_t* n,
+// Builds the batch's groups and uploads their descriptors.  src[t] is the
+// device address of instance t's rows.  The descriptors go up from pinned
+// staging owned by the context, asynchronously: the host does not wait, and
+// the staging is reused once the previous batch's copies have completed.
+int batch_setup(sky_ctx* c, const uint64_t* src, uint32_t n_inst, const uint32_t* n,
                 const uint32_t* d, const uint32_t* max_d) {
   free_batch(c);
   BatchState& B = c->batch;
   B.n_inst = n_inst;
-  std::vector<uint64_t> foff(n_inst + 1, 0);
-  for (uint32_t t = 0; t < n_inst; ++t) foff[t + 1] = foff[t] + (uint64_t)n[t] * d[t];
+  // descriptor bytes of every group, 16-byte aligned parts
+  auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
+  uint32_t cnt[17] = {0};
+  uint64_t ntask[17] = {0};
+  for (uint32_t t = 0; t < n_inst; ++t)
+    if (n[t] > 0) {
+      cnt[d[t]]++;
+      ntask[d[t]] += (n[t] + kBlock - 1) / kBlock;
+    }
+  size_t stage_bytes = 0;
+  for (int D = 1; D <= 16; ++D)
+    if (cnt[D])
+      stage_bytes += up16(4ull * (cnt[D] + 1)) + up16(8ull * cnt[D]) + up16(8ull * ntask[D]) +
+                     up16(4ull * cnt[D]);
+  if (B.desc_done) HIPCHK(hipEventSynchronize(B.desc_done));  // staging free again
+  else HIPCHK(hipEventCreateWithFlags(&B.desc_done, hipEventDisableTiming));
+  if (stage_bytes > B.desc_cap) {
+    if (B.desc_host) HIPCHK(hipHostFree(B.desc_host));
+    B.desc_host = nullptr;
+    B.desc_cap = 0;
+    const size_t want = std::max<size_t>(stage_bytes + stage_bytes / 2, 64u << 10);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&B.desc_host), want, hipHostMallocDefault));
+    B.desc_cap = want;
+  }
+  uint8_t* sp = B.desc_host;
+  auto stage = [&](size_t bytes) {
+    uint8_t* p = sp;
+    sp += up16(bytes);
+    return p;
+  };
   uint32_t maxN = 0, maxT = 0;
   for (int D = 1; D <= 16; ++D) {
     BatchGroup& g = B.pool[D];
     g.D = D;
     g.inst.clear();
+    if (!cnt[D]) continue;
     for (uint32_t t = 0; t < n_inst; ++t)
       if ((int)d[t] == D && n[t] > 0) g.inst.push_back(t);
-    if (g.inst.empty()) continue;
     g.T = (uint32_t)g.inst.size();
     g.off.assign(g.T + 1, 0);
-    std::vector<uint64_t> sf(g.T);
-    std::vector<uint32_t> md(g.T);
-    std::vector<uint2> tasks;
+    uint32_t* h_off = reinterpret_cast<uint32_t*>(stage(4ull * (g.T + 1)));
+    uint64_t* h_src = reinterpret_cast<uint64_t*>(stage(8ull * g.T));
+    uint2* h_tasks = reinterpret_cast<uint2*>(stage(8ull * ntask[D]));
+    uint32_t* h_md = reinterpret_cast<uint32_t*>(stage(4ull * g.T));
+    uint32_t nt = 0;
     for (uint32_t k = 0; k < g.T; ++k) {
       const uint32_t t = g.inst[k];
       if ((uint64_t)g.off[k] + n[t] > 0xFFFFFFFFull)
@@ -503,29 +545,30 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
       if (n[t] >= (1u << 29))
         return fail(c, SKY_ERR_CAPACITY, "instance %u: %u points (limit 2^29 - 1)", t, n[t]);
       g.off[k + 1] = g.off[k] + n[t];
-      sf[k] = foff[t];
-      md[k] = (max_d && max_d[t] && max_d[t] < (uint32_t)D) ? max_d[t] : (uint32_t)D;
-      for (uint32_t s0 = 0; s0 < n[t]; s0 += kBlock) tasks.push_back(make_uint2(k, g.off[k] + s0));
+      h_src[k] = src[t];
+      h_md[k] = (max_d && max_d[t] && max_d[t] < (uint32_t)D) ? max_d[t] : (uint32_t)D;
+      for (uint32_t s0 = 0; s0 < n[t]; s0 += kBlock) h_tasks[nt++] = make_uint2(k, g.off[k] + s0);
     }
+    std::copy(g.off.begin(), g.off.end(), h_off);
     g.N = g.off[g.T];
-    g.ntasks = (uint32_t)tasks.size();
+    g.ntasks = nt;
     TRY(ensure(c, g.d_off, sizeof(uint32_t) * (g.T + 1)));
-    TRY(ensure(c, g.d_src_foff, sizeof(uint64_t) * g.T));
-    TRY(ensure(c, g.d_tasks, sizeof(uint2) * tasks.size()));
+    TRY(ensure(c, g.d_src, sizeof(uint64_t) * g.T));
+    TRY(ensure(c, g.d_tasks, sizeof(uint2) * nt));
     TRY(ensure(c, g.d_maxd, sizeof(uint32_t) * g.T));
     TRY(ensure(c, g.ext_pids, sizeof(uint32_t) * g.N));
     TRY(ensure(c, g.eoff, sizeof(uint32_t) * (g.T + 1)));
     TRY(ensure(c, g.dom, sizeof(uint32_t) * (size_t)g.N * sky_words_per_point(D)));
     TRY(ensure(c, g.cnt, sizeof(uint32_t) * kGroupCounters));
-    HIPCHK(hipMemcpyAsync(g.d_off.p, g.off.data(), sizeof(uint32_t) * (g.T + 1),
-                          hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(g.d_src_foff.p, sf.data(), sizeof(uint64_t) * g.T,
-                          hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(g.d_tasks.p, tasks.data(), sizeof(uint2) * tasks.size(),
-                          hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(g.d_maxd.p, md.data(), sizeof(uint32_t) * g.T, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(g.d_off.p, h_off, sizeof(uint32_t) * (g.T + 1), hipMemcpyHostToDevice,
                           c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    HIPCHK(hipMemcpyAsync(g.d_src.p, h_src, sizeof(uint64_t) * g.T, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(g.d_tasks.p, h_tasks, sizeof(uint2) * nt, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(g.d_maxd.p, h_md, sizeof(uint32_t) * g.T, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipEventRecord(B.desc_done, c->stream));
     maxN = std::max(maxN, g.N);
     maxT = std::max(maxT, g.T);
     B.groups.push_back(&g);
@@ -534,7 +577,7 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   for (const BatchGroup* g : B.groups)
     if (batch_self_index(g->D, g->T)) tab = std::max<uint64_t>(tab, ((uint64_t)g->T << g->D) + 1);
   TRY(batch_buffers(c, maxN, maxT, tab));
-  B.sig_rows = rows_dev;
+  B.sig_src.assign(src, src + n_inst);
   B.sig_tune = c->tune_ver;
   B.sig_n.assign(n, n + n_inst);
   B.sig_d.assign(d, d + n_inst);
@@ -545,10 +588,10 @@ int batch_setup(sky_ctx* c, const float* rows_dev, uint32_t n_inst, const uint32
   return SKY_OK;
 }
 
-int batch_enqueue(sky_ctx* c, const float* rows_dev) {
+int batch_enqueue(sky_ctx* c) {
   for (BatchGroup* g : c->batch.groups) {
     int rc = SKY_OK;
-    SKY_DISPATCH_D(g->D, rc = batch_group_d<D>(c, *g, rows_dev));
+    SKY_DISPATCH_D(g->D, rc = batch_group_d<D>(c, *g));
     if (rc != SKY_OK) return rc;
   }
   return SKY_OK;
@@ -626,11 +669,14 @@ int batch_copy_out(sky_ctx* c, uint32_t* const* ext_dst, uint32_t* const* dom_ds
   return SKY_OK;
 }
 
-bool batch_same(const BatchState& B, const float* rows, uint32_t n_inst, const uint32_t* n,
+int batch_finish(sky_ctx* c, uint64_t* total_ext);
+
+bool batch_same(const BatchState& B, const uint64_t* src, uint32_t n_inst, const uint32_t* n,
                 const uint32_t* d, const uint32_t* max_d) {
-  if (!B.ready || B.sig_rows != rows || B.n_inst != n_inst) return false;
+  if (!B.ready || B.n_inst != n_inst) return false;
   for (uint32_t t = 0; t < n_inst; ++t)
-    if (B.sig_n[t] != n[t] || B.sig_d[t] != d[t] || B.sig_m[t] != (max_d ? max_d[t] : 0u))
+    if (B.sig_src[t] != src[t] || B.sig_n[t] != n[t] || B.sig_d[t] != d[t] ||
+        B.sig_m[t] != (max_d ? max_d[t] : 0u))
       return false;
   return true;
 }
@@ -750,7 +796,6 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
   uint64_t tot = 0;
   // a first pass builds the batch and loads every kernel
   TRY(sky_batch_mdmc(c, rows, rows_on_device, n_inst, n, d, max_d, 0, &tot));
-  const float* rows_dev = rows_on_device ? rows : P<float>(c->rows);
   BatchState& B = c->batch;
   const size_t G = B.groups.size();
   std::vector<hipEvent_t> ev(2 * G, nullptr);
@@ -778,7 +823,7 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
       for (size_t gi = 0; gi < G && rc == SKY_OK; ++gi) {
         BatchGroup& g = *B.groups[gi];
         if (hipEventRecord(ev[2 * gi], c->stream) != hipSuccess) rc = SKY_ERR_DEVICE;
-        if (rc == SKY_OK) SKY_DISPATCH_D(g.D, rc = batch_group_d<D>(c, g, rows_dev));
+        if (rc == SKY_OK) SKY_DISPATCH_D(g.D, rc = batch_group_d<D>(c, g));
         if (rc == SKY_OK && hipEventRecord(ev[2 * gi + 1], c->stream) != hipSuccess)
           rc = SKY_ERR_DEVICE;
       }
@@ -955,13 +1000,19 @@ int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n
                             c->stream));
     rows_dev = P<float>(c->rows);
   }
+  std::vector<uint64_t> src(n_inst);
+  uint64_t f = 0;
+  for (uint32_t t = 0; t < n_inst; ++t) {
+    src[t] = reinterpret_cast<uint64_t>(rows_dev + f);
+    f += (uint64_t)n[t] * d[t];
+  }
   BatchState& B = c->batch;
-  if (!batch_same(B, rows_dev, n_inst, n, d, max_d) || B.sig_tune != c->tune_ver)
-    TRY(batch_setup(c, rows_dev, n_inst, n, d, max_d));
+  if (!batch_same(B, src.data(), n_inst, n, d, max_d) || B.sig_tune != c->tune_ver)
+    TRY(batch_setup(c, src.data(), n_inst, n, d, max_d));
   if (use_graph) {
     if (!B.exec) {
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = batch_enqueue(c, rows_dev);
+      const int rc = batch_enqueue(c);
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(c->stream, &graph);
       if (rc != SKY_OK) {
@@ -980,9 +1031,43 @@ int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n
     }
     HIPCHK(hipGraphLaunch(B.exec, c->stream));
   } else {
-    TRY(batch_enqueue(c, rows_dev));
+    TRY(batch_enqueue(c));
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
+  return batch_finish(c, total_ext);
+}
+
+int sky_batch_mdmc_dev(sky_ctx* c, const float* const* inst_rows_dev, uint32_t n_inst,
+                       const uint32_t* n, const uint32_t* d, const uint32_t* max_d,
+                       uint64_t* total_ext) {
+  TRY(check_ctx(c));
+  if (!n || !d || !total_ext || (n_inst && !inst_rows_dev))
+    return fail(c, SKY_ERR_ARG, "null argument");
+  std::vector<uint64_t> src(n_inst);
+  for (uint32_t t = 0; t < n_inst; ++t) {
+    if (d[t] < 1 || d[t] > 16) return fail(c, SKY_ERR_ARG, "instance %u: d=%u outside [1,16]", t, d[t]);
+    if (n[t] && !inst_rows_dev[t]) return fail(c, SKY_ERR_ARG, "instance %u: null rows", t);
+    src[t] = reinterpret_cast<uint64_t>(inst_rows_dev[t]);
+  }
+  c->has_mdmc = false;
+  c->has_hc = false;
+  c->has_batch = false;
+  BatchState& B = c->batch;
+  if (!batch_same(B, src.data(), n_inst, n, d, max_d) || B.sig_tune != c->tune_ver)
+    TRY(batch_setup(c, src.data(), n_inst, n, d, max_d));
+  TRY(batch_enqueue(c));
+  return batch_finish(c, total_ext);
+}
+
+void* sky_ctx_stream(sky_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+}  // extern "C"
+
+namespace {
+
+// Waits for the batch and reads its counters: the NaN flag and the
+// extended-skyline sizes of every group.
+int batch_finish(sky_ctx* c, uint64_t* total_ext) {
+  BatchState& B = c->batch;
   uint64_t tot = 0;
   std::vector<uint32_t> h(kGroupCounters * B.groups.size());
   for (size_t gi = 0; gi < B.groups.size(); ++gi)
@@ -997,6 +1082,10 @@ int sky_batch_mdmc(sky_ctx* c, const float* rows, int rows_on_device, uint32_t n
   c->has_batch = true;
   return SKY_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int sky_batch_fetch(sky_ctx* c, uint64_t* ext_off, uint32_t* ext_pids, uint64_t* dom_off,
                     uint32_t* dom_words, uint64_t cap_ext, uint64_t cap_words) {

@@ -31,14 +31,17 @@ struct BatchGroup {
   std::vector<uint32_t> inst;  // caller's instance ids, in group order
   std::vector<uint32_t> off;   // point offsets of the instances (T + 1)
   uint32_t T = 0, N = 0, ntasks = 0;
-  DevBuf d_off, d_src_foff, d_tasks, d_maxd;  // descriptors
+  DevBuf d_off, d_src, d_tasks, d_maxd;       // descriptors (d_src: row address per instance)
   DevBuf ext_pids, eoff, dom, cnt;            // results; cnt: counters
   mutable std::vector<uint32_t> eo_host;       // eoff read back by the last fetch
 };
 
 struct BatchState {
   std::vector<uint32_t> sig_n, sig_d, sig_m;
-  const float* sig_rows = nullptr;
+  std::vector<uint64_t> sig_src;    // row address of every instance
+  uint8_t* desc_host = nullptr;     // pinned staging of the descriptors
+  size_t desc_cap = 0;
+  hipEvent_t desc_done = nullptr;   // the staging's last copy
   BatchGroup pool[17];              // per dimensionality, buffers kept across batches
   std::vector<BatchGroup*> groups;  // the groups of the current batch, ascending d
   uint32_t n_inst = 0;
@@ -260,7 +263,7 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey);
 
 // One batch group's whole pass, enqueued on the context stream.
 template <int D>
-int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev);
+int batch_group_d(sky_ctx* c, BatchGroup& g);
 
 // One chunk of an SDSC lattice level: T cuboids of D columns (subs[t]), each
 // over its candidates cand[off[t] .. off[t+1]) (row ids of the n x d rows,

@@ -1,19 +1,24 @@
 // server.cpp -- continuous-batching skycube server over the C ABI.
 //
 // Requests (one skycube each: rows, d, max_d) are queued by sky_server_submit
-// from any thread.  Worker threads, each with its own device context and
+// from any thread.  At submission the rows are copied into pinned host
+// memory and their upload to device memory is enqueued at once, on the
+// server's upload stream, into a stream-ordered memory pool: a request's rows
+// are resident in HBM while it waits, and the copy overlaps whatever the
+// device is running.  Worker threads, each with its own device context and
 // stream, repeatedly take every queued request that fits the HBM budget (the
 // estimated device footprint of the requests in flight, first come first
-// served) and run them as one batched MDMC pass (sky_batch_mdmc).  A request
+// served) and run them as one batched MDMC pass over the resident rows
+// (sky_batch_mdmc_dev): the worker's stream waits on the requests' upload
+// events, the batch descriptors go up asynchronously, and the only host
+// waits are the ones that read the result sizes and the results.  A request
 // that arrives while the device is busy joins the next batch, and one
-// worker's host work (row upload, set-up, result download) overlaps the
-// other's kernels.  Each worker owns hbm_budget / workers bytes of device
-// memory; a batch is admitted only while the estimated footprint of its
-// requests fits that share.  Rows go up from a pinned staging buffer per
-// worker; results come down straight from the batch's device buffers into
-// pinned host memory owned by the request (an arena of pinned slabs) until
-// sky_server_result copies them out.  The reference has no counterpart (it
-// runs one skycube per process).
+// worker's host work overlaps the others' kernels.  Each worker owns an
+// equal share of the budget (less the part reserved for resident rows) and
+// keeps its device buffers across batches.  Results come down straight from
+// the batch's device buffers into pinned host memory owned by the request
+// (an arena of pinned slabs) until sky_server_result copies them out.  The
+// reference has no counterpart (it runs one skycube per process).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,7 +97,14 @@ class PinnedArena {
 
 struct Request {
   uint64_t id = 0;
-  std::vector<float> rows;
+  // rows on the host: a pinned region (rows_slab >= 0) or rows_v
+  float* rows = nullptr;
+  int rows_slab = -1;
+  std::vector<float> rows_v;
+  // rows in device memory once the upload is enqueued, and its event
+  float* dev = nullptr;
+  hipEvent_t up = nullptr;
+  uint64_t resident = 0;  // bytes counted against the resident-row reserve
   uint32_t n = 0, d = 0, max_d = 0;
   uint64_t footprint = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
@@ -109,29 +121,6 @@ struct Request {
   std::vector<uint32_t> ext_v, dom_v;
 };
 
-// Pinned staging buffer for the rows of one worker's batch.
-struct Staging {
-  float* p = nullptr;
-  size_t cap = 0;
-  ~Staging() {
-    if (p) (void)hipHostFree(p);
-  }
-  float* get(size_t floats) {
-    if (floats > cap) {
-      if (p) (void)hipHostFree(p);
-      p = nullptr;
-      cap = 0;
-      void* q = nullptr;
-      const size_t want = std::max<size_t>(floats + floats / 4, 1u << 20);
-      if (hipHostMalloc(&q, want * sizeof(float), hipHostMallocDefault) != hipSuccess)
-        return nullptr;
-      p = static_cast<float*>(q);
-      cap = want;
-    }
-    return p;
-  }
-};
-
 }  // namespace
 
 struct sky_server {
@@ -145,6 +134,10 @@ struct sky_server {
   std::vector<sky_ctx*> ctxs;
   std::vector<std::thread> workers;
   PinnedArena arena;
+  int device = 0;
+  hipStream_t up_stream = nullptr;  // uploads of submitted rows
+  hipMemPool_t pool = nullptr;      // device memory of resident rows
+  uint64_t resident = 0, resident_cap = 0;
   uint64_t batches = 0, max_concurrent = 0, concurrent = 0;
   double busy_s = 0, busy_since = 0;  // time with at least one batch running
 };
@@ -167,35 +160,55 @@ namespace {
 // offsets of the sixteen dimensionality groups, sort temporaries).
 constexpr uint64_t kBatchFixed = 4ull << 20;
 
-void run_batch(sky_server* s, sky_ctx* ctx, Staging& stage, std::vector<Request*>& take,
-               uint64_t need) {
+// Device bytes of a request's rows.
+size_t row_bytes(const Request* r) { return sizeof(float) * (size_t)r->n * r->d; }
+
+// Gives back a request's rows: the device copy (stream-ordered on `st`, after
+// the work that read it), the upload event, the host copy.
+void drop_rows(sky_server* s, Request* r, hipStream_t st) {
+  if (r->dev) (void)hipFreeAsync(r->dev, st);
+  r->dev = nullptr;
+  if (r->up) (void)hipEventDestroy(r->up);
+  r->up = nullptr;
+  s->arena.release(r->rows_slab);
+  r->rows_slab = -1;
+  r->rows = nullptr;
+  r->rows_v.clear();
+  r->rows_v.shrink_to_fit();
+}
+
+void run_batch(sky_server* s, sky_ctx* ctx, std::vector<Request*>& take, uint64_t need) {
   const size_t B = take.size();
+  hipStream_t st = static_cast<hipStream_t>(sky_ctx_stream(ctx));
   std::vector<uint32_t> n(B), d(B), m(B);
-  size_t total = 0;
-  for (size_t k = 0; k < B; ++k) {
-    n[k] = take[k]->n;
-    d[k] = take[k]->d;
-    m[k] = take[k]->max_d;
-    total += take[k]->rows.size();
-  }
+  std::vector<const float*> src(B);
   // buffers kept from earlier batches count against the share: drop them
   // when they and this batch together might not fit
   int rc = SKY_OK;
   if (sky_ctx_bytes_in_use(ctx) + need > s->share) rc = sky_ctx_release(ctx);
-  std::vector<float> pageable;
-  float* rows = stage.get(std::max<size_t>(total, 1));
-  if (!rows) {
-    pageable.resize(std::max<size_t>(total, 1));
-    rows = pageable.data();
-  }
-  size_t o = 0;
-  for (Request* r : take) {
-    std::copy(r->rows.begin(), r->rows.end(), rows + o);
-    o += r->rows.size();
+  for (size_t k = 0; k < B && rc == SKY_OK; ++k) {
+    Request* r = take[k];
+    n[k] = r->n;
+    d[k] = r->d;
+    m[k] = r->max_d;
+    const size_t bytes = row_bytes(r);
+    if (r->dev) {  // resident: order the batch after its upload
+      if (hipStreamWaitEvent(st, r->up, 0) != hipSuccess) rc = SKY_ERR_DEVICE;
+    } else if (bytes) {  // not resident (reserve full): upload on this stream
+      void* p = nullptr;
+      if (hipMallocFromPoolAsync(&p, bytes, s->pool, st) != hipSuccess) {
+        rc = SKY_ERR_OOM_BUDGET;
+      } else {
+        r->dev = static_cast<float*>(p);
+        if (hipMemcpyAsync(r->dev, r->rows, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+          rc = SKY_ERR_DEVICE;
+      }
+    }
+    src[k] = r->dev;
   }
   uint64_t tot = 0;
   if (rc == SKY_OK)
-    rc = sky_batch_mdmc(ctx, rows, 0, (uint32_t)B, n.data(), d.data(), m.data(), 0, &tot);
+    rc = sky_batch_mdmc_dev(ctx, src.data(), (uint32_t)B, n.data(), d.data(), m.data(), &tot);
   std::vector<uint64_t> eo(B + 1), dof(B + 1);
   if (rc == SKY_OK) rc = sky_batch_fetch(ctx, eo.data(), nullptr, dof.data(), nullptr, 0, 0);
   if (rc == SKY_OK) {
@@ -221,14 +234,16 @@ void run_batch(sky_server* s, sky_ctx* ctx, Staging& stage, std::vector<Request*
     }
     rc = sky_batch_fetch_into(ctx, ed.data(), dd.data());
   }
-  const std::string err = rc == SKY_OK ? "" : sky_last_error(ctx);
+  const std::string err =
+      rc == SKY_OK ? "" : (rc == SKY_ERR_OOM_BUDGET ? "no device memory for the rows" : sky_last_error(ctx));
   const double t = now_s();
   std::lock_guard<std::mutex> lk(s->mu);
   for (Request* r : take) {
+    drop_rows(s, r, st);
+    s->resident -= r->resident;
+    r->resident = 0;
     r->rc = rc;
     r->err = err;
-    r->rows.clear();
-    r->rows.shrink_to_fit();
     r->t_done = t;
     r->state = 2;
     r->batch_size = (uint32_t)B;
@@ -246,7 +261,7 @@ void run_batch(sky_server* s, sky_ctx* ctx, Staging& stage, std::vector<Request*
 
 void worker(sky_server* s, size_t w) {
   sky_ctx* ctx = s->ctxs[w];
-  Staging stage;
+  (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
@@ -273,9 +288,22 @@ void worker(sky_server* s, size_t w) {
     if (s->concurrent++ == 0) s->busy_since = now_s();
     s->max_concurrent = std::max(s->max_concurrent, s->concurrent);
     lk.unlock();
-    run_batch(s, ctx, stage, take, need);
+    run_batch(s, ctx, take, need);
     lk.lock();
   }
+}
+
+// Frees the server's contexts, resident-row pool and upload stream.
+void release_device(sky_server* s) {
+  for (sky_ctx* c : s->ctxs) sky_ctx_destroy(c);
+  s->ctxs.clear();
+  if (s->up_stream) {
+    (void)hipStreamSynchronize(s->up_stream);
+    (void)hipStreamDestroy(s->up_stream);
+  }
+  s->up_stream = nullptr;
+  if (s->pool) (void)hipMemPoolDestroy(s->pool);
+  s->pool = nullptr;
 }
 
 }  // namespace
@@ -298,16 +326,36 @@ int sky_server_create(int device, size_t hbm_budget_bytes, uint32_t workers, uin
     hbm_budget_bytes = free_b / 10 * 8;
   }
   s->budget = hbm_budget_bytes;
-  s->share = hbm_budget_bytes / workers;
+  s->device = device;
+  // 1/16 of the budget holds the rows of waiting requests; the rest is
+  // split evenly over the workers
+  s->resident_cap = hbm_budget_bytes / 16;
+  s->share = (hbm_budget_bytes - s->resident_cap) / workers;
   if (s->share <= kBatchFixed) return SKY_ERR_OOM_BUDGET;
-  for (uint32_t w = 0; w < workers; ++w) {
+  int rc = hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking) == hipSuccess
+               ? SKY_OK : SKY_ERR_DEVICE;
+  if (rc == SKY_OK) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    uint64_t keep = ~0ull;  // freed blocks stay in the pool for the next requests
+    int no = 0;             // reuse never makes one stream wait for another
+    if (hipMemPoolCreate(&s->pool, &props) != hipSuccess ||
+        hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess ||
+        hipMemPoolSetAttribute(s->pool, hipMemPoolReuseAllowInternalDependencies, &no) !=
+            hipSuccess)
+      rc = SKY_ERR_DEVICE;
+  }
+  for (uint32_t w = 0; w < workers && rc == SKY_OK; ++w) {
     sky_ctx* c = nullptr;
-    const int rc = sky_ctx_create(device, s->share, &c);
-    if (rc != SKY_OK) {
-      for (sky_ctx* x : s->ctxs) sky_ctx_destroy(x);
-      return rc;
-    }
-    s->ctxs.push_back(c);
+    rc = sky_ctx_create(device, s->share, &c);
+    if (rc == SKY_OK) s->ctxs.push_back(c);
+  }
+  if (rc != SKY_OK) {
+    release_device(s.get());
+    return rc;
   }
   for (uint32_t w = 0; w < workers; ++w) s->workers.emplace_back(worker, s.get(), (size_t)w);
   *out = s.release();
@@ -323,7 +371,7 @@ void sky_server_destroy(sky_server* s) {
   s->cv_work.notify_all();
   s->cv_done.notify_all();
   for (std::thread& t : s->workers) t.join();
-  for (sky_ctx* c : s->ctxs) sky_ctx_destroy(c);
+  release_device(s);
   delete s;
 }
 
@@ -334,13 +382,61 @@ int sky_server_submit(sky_server* s, const float* rows, uint32_t n, uint32_t d, 
   r->n = n;
   r->d = d;
   r->max_d = max_d;
-  r->rows.assign(rows, rows + (size_t)n * d);
-  for (float v : r->rows)
-    if (std::isnan(v)) return SKY_ERR_NAN;
   r->footprint = sky_server_footprint(n, d);
   r->t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->footprint + kBatchFixed > s->share) return SKY_ERR_OOM_BUDGET;
+  }
+  const size_t count = (size_t)n * d, bytes = sizeof(float) * count;
+  for (size_t i = 0; i < count; ++i)
+    if (std::isnan(rows[i])) return SKY_ERR_NAN;
+  if (bytes) {
+    r->rows = static_cast<float*>(s->arena.alloc(bytes, &r->rows_slab));
+    if (!r->rows) {  // no pinned memory left: pageable
+      r->rows_v.assign(rows, rows + count);
+      r->rows = r->rows_v.data();
+    } else {
+      std::memcpy(r->rows, rows, bytes);
+    }
+    // reserve room among the resident rows, then enqueue the upload
+    bool room = false;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->resident + bytes <= s->resident_cap) {
+        s->resident += bytes;
+        r->resident = bytes;
+        room = true;
+      }
+    }
+    if (room) {
+      int prev = 0;
+      const bool dev_ok = hipGetDevice(&prev) == hipSuccess && hipSetDevice(s->device) == hipSuccess;
+      void* p = nullptr;
+      bool ok = dev_ok && hipMallocFromPoolAsync(&p, bytes, s->pool, s->up_stream) == hipSuccess;
+      if (ok) {
+        r->dev = static_cast<float*>(p);
+        ok = hipMemcpyAsync(r->dev, r->rows, bytes, hipMemcpyHostToDevice, s->up_stream) ==
+                 hipSuccess &&
+             hipEventCreateWithFlags(&r->up, hipEventDisableTiming) == hipSuccess &&
+             hipEventRecord(r->up, s->up_stream) == hipSuccess;
+      }
+      if (!ok) {  // the worker uploads the rows instead
+        if (r->up) (void)hipEventDestroy(r->up);
+        r->up = nullptr;
+        if (r->dev) {
+          (void)hipStreamSynchronize(s->up_stream);
+          (void)hipFreeAsync(r->dev, s->up_stream);
+        }
+        r->dev = nullptr;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->resident -= bytes;
+        r->resident = 0;
+      }
+      if (dev_ok) (void)hipSetDevice(prev);
+    }
+  }
   std::lock_guard<std::mutex> lk(s->mu);
-  if (r->footprint + kBatchFixed > s->share) return SKY_ERR_OOM_BUDGET;
   r->id = s->next_id++;
   *id = r->id;
   s->queue.push_back(r.get());

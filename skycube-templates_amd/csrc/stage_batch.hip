@@ -76,7 +76,7 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
 
 // Enqueues one group's whole pass on the context stream.
 template <int D>
-int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
+int batch_group_d(sky_ctx* c, BatchGroup& g) {
   BatchState& B = c->batch;
   const uint32_t N = g.N, T = g.T;
   uint32_t* cnt = P<uint32_t>(g.cnt);  // [0] err [1] survivors [2] ext ids [3] ext score [4] work
@@ -85,8 +85,8 @@ int batch_group_d(sky_ctx* c, BatchGroup& g, const float* rows_dev) {
   const int tbits = tbits_of(T);
   TRY(fill32(c, cnt, kGroupCounters, 0));
   TRY(fill8(c, B.flag_pt.p, N, 0));
-  hipLaunchKernelGGL(sky::k_bcopy<D>, dim3(gN), dim3(kBlock), 0, c->stream, rows_dev,
-                     P<uint64_t>(g.d_src_foff), off, T, N, P<float>(B.rows));
+  hipLaunchKernelGGL(sky::k_bcopy<D>, dim3(gN), dim3(kBlock), 0, c->stream,
+                     P<uint64_t>(g.d_src), off, T, N, P<float>(B.rows));
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_bscore<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.rows), off,
                      T, N, P<uint64_t>(B.key_a), P<uint32_t>(B.val_a), cnt);
@@ -295,7 +295,7 @@ int sdsc_chunk_d(sky_ctx* c, const SdscChunk& k) {
   return SKY_OK;
 }
 
-template int batch_group_d<SKY_D>(sky_ctx*, BatchGroup&, const float*);
+template int batch_group_d<SKY_D>(sky_ctx*, BatchGroup&);
 template int sdsc_chunk_d<SKY_D>(sky_ctx*, const SdscChunk&);
 
 }  // namespace skyi

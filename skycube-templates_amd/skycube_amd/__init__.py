@@ -47,7 +47,8 @@ EXPORTS = (
     "sky_batch_fetch_into", "sky_ctx_get_option", "sky_batch_autotune",
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
-    "sky_mdmc_device_noalloc", "sky_sdsc_device", "sky_sdsc_fetch",
+    "sky_mdmc_device_noalloc", "sky_sdsc_device", "sky_sdsc_fetch", "sky_batch_mdmc_dev",
+    "sky_ctx_stream",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 
@@ -76,6 +77,13 @@ def load():
         raise FileNotFoundError(
             f"{_LIB_PATH} is missing: build it with `make -C {_PKG_ROOT}` "
             "(or __graft_entry__.build())")
+    # torch, where installed, ships its own HIP runtime; loaded first, it is
+    # the runtime the library binds too, so torch and the library share one
+    # runtime in the process whichever the caller imports first
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(_LIB_PATH)
     L.sky_ctx_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
     L.sky_ctx_destroy.argtypes = [ctypes.c_void_p]
@@ -110,6 +118,10 @@ def load():
                                      ctypes.c_uint64, ctypes.c_uint64]
     L.sky_batch_mdmc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
                                   _u32p, _u32p, _u32p, ctypes.c_int, _u64p]
+    L.sky_batch_mdmc_dev.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.c_uint32, _u32p, _u32p, _u32p, _u64p]
+    L.sky_ctx_stream.argtypes = [ctypes.c_void_p]
+    L.sky_ctx_stream.restype = ctypes.c_void_p
     L.sky_batch_fetch.argtypes = [ctypes.c_void_p, _u64p, _u32p, _u64p, _u32p, ctypes.c_uint64,
                                   ctypes.c_uint64]
     L.sky_batch_fetch_into.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
@@ -401,6 +413,25 @@ class Context:
                                            int(bool(use_graph)), _ptr(tot, _u64p)))
         return int(tot[0])
 
+    def batch_mdmc_dev(self, rows_dev_ptrs, n, d, max_d=None) -> int:
+        """Runs a batch whose instances live in separate device allocations
+        (one device pointer per instance); returns the total number of
+        extended-skyline points (results stay on device)."""
+        n = np.ascontiguousarray(n, dtype=np.uint32)
+        d = np.ascontiguousarray(d, dtype=np.uint32)
+        m = None if max_d is None else np.ascontiguousarray(max_d, dtype=np.uint32)
+        ptrs = (ctypes.c_void_p * max(len(rows_dev_ptrs), 1))(*[int(p) for p in rows_dev_ptrs])
+        tot = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.sky_batch_mdmc_dev(self._h, ptrs, n.size, _ptr(n, _u32p),
+                                               _ptr(d, _u32p),
+                                               None if m is None else _ptr(m, _u32p),
+                                               _ptr(tot, _u64p)))
+        return int(tot[0])
+
+    def stream(self) -> int:
+        """The context's HIP stream, as an integer handle."""
+        return int(self._L.sky_ctx_stream(self._h) or 0)
+
     def batch_fetch(self, n_inst: int):
         """[(ext_pids, bitmaps)] of the last batch, in instance order."""
         eo = np.zeros(n_inst + 1, dtype=np.uint64)
@@ -492,7 +523,7 @@ class Server:
     ``result`` blocks until it is done and returns (ext_pids, dom, latency_ms,
     batch_size), the same arrays ``Context.mdmc`` returns for those rows."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2,
+    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 4,
                  max_batch: int = 0):
         self._lib = load()
         h = ctypes.c_void_p()

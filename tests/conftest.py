@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# torch ships its own HIP runtime.  Importing it before the library is loaded
+# makes the library bind that same runtime, so tests that hand torch device
+# tensors to the library run on one runtime in the process.
+try:
+    import torch  # noqa: F401
+except ImportError:  # CPU-only checks do not need it
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "skycube-templates_amd")):
     if p not in sys.path:

@@ -142,3 +142,26 @@ def test_batch_wide_and_narrow_d(ctx):
                 got = c.batch_mdmc(mats, [m or 0 for m in caps])
                 for k, ((e1, d1), (e2, d2)) in enumerate(zip(alone, got)):
                     assert np.array_equal(e1, e2) and np.array_equal(d1, d2), (tile, order, k)
+
+
+def test_batch_dev_separate_allocations(ctx):
+    """Instances in separate device tensors (sky_batch_mdmc_dev) give the
+    concatenated batch's results; two calls in a row, the second smaller and
+    with other pointers, so the asynchronously staged descriptors of the
+    first are not reused by mistake."""
+    import torch
+    mats = _instances(30, 21, n_hi=2500)
+    caps = [max(1, m.shape[1] - (k % 4)) for k, m in enumerate(mats)]
+    want = ctx.batch_mdmc(mats, caps)
+    for rnd, sel in enumerate([list(range(30)), list(range(3, 30, 3))]):
+        print(f"dev batch round {rnd}: {len(sel)} instances", flush=True)
+        tens = [torch.from_numpy(mats[k]).to("cuda:0") for k in sel]
+        torch.cuda.synchronize()
+        tot = ctx.batch_mdmc_dev([t.data_ptr() for t in tens], [mats[k].shape[0] for k in sel],
+                                 [mats[k].shape[1] for k in sel], [caps[k] for k in sel])
+        got = ctx.batch_fetch(len(sel))
+        assert tot == sum(int(want[k][0].size) for k in sel)
+        for j, k in enumerate(sel):
+            assert np.array_equal(got[j][0], want[k][0]), (rnd, k)
+            assert np.array_equal(got[j][1], want[k][1]), (rnd, k)
+        del tens

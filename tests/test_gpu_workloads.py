@@ -98,7 +98,7 @@ def cfg5():
     ids = [None] * len(mats)
     bsz = [0] * len(mats)
     errors = []
-    with skycube_amd.Server(0, workers=2) as srv:
+    with skycube_amd.Server(0, workers=4) as srv:
         def collect():
             try:
                 for k in range(len(mats)):
