@@ -69,13 +69,13 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
 /* Tuning options of a context (results never depend on them).  No
  * reference counterpart (the reference's kernels have fixed launch shapes).
- *   SKY_OPT_MEMBER_INDEX  how the membership kernel visits candidates: 0
- *                         in score order; 1 through median-mask buckets; 2
- *                         in groups of 64 of the code order, skipped whole
- *                         by their column-minimum codes; -1 (default): 2 in
- *                         the single-instance path, and in the batched path
- *                         1 when the buckets hold SKY_OPT_MIN_BUCKET or
- *                         more candidates on average, else 0
+ *   SKY_OPT_MEMBER_INDEX  how the membership kernel of the single-instance
+ *                         path visits candidates: 0 in score order; 1
+ *                         through median-mask buckets; 2 in groups of 64 of
+ *                         the code order, skipped whole by their
+ *                         column-minimum codes; 3 the same groups narrowed
+ *                         to their live subgroups of 8 first; -1 (default):
+ *                         3 up to d = 14, 2 above
  *   SKY_OPT_COVER         lowest-norm candidates covered before the
  *                         coverage pruning starts (64 .. 65536; 0, the
  *                         default: about n_ext / 32 as a power of two in
@@ -89,9 +89,10 @@ size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
  *                         kernel (2048; 64 .. 8192)
  *   SKY_OPT_TILE          points per membership workgroup in the batched
  *                         path: 0 the default, 1 half of it
- *   SKY_OPT_BATCH_ORDER   candidate order of the batched membership: 2
- *                         (default) code-order groups, 0 scan or buckets
- *                         as SKY_OPT_MIN_BUCKET decides
+ *   SKY_OPT_BATCH_ORDER   candidate order of the batched membership: 3
+ *                         (default) code-order groups and subgroups, 2
+ *                         code-order groups, 0 scan or buckets as
+ *                         SKY_OPT_MIN_BUCKET decides
  * option | (d << 8) sets the value for dimensionality d only (d = 1..16);
  * a plain option sets it for every d.  sky_ctx_get_option reads one back
  * (for d = 1 when no d is given). */
@@ -159,8 +160,8 @@ int sky_mdmc_device(sky_ctx *ctx, const float *rows_dev, uint32_t n, uint32_t d,
  * bitmaps sky_mdmc_device would leave on the device, *n_ext their count --
  * and every other buffer is carved from a 64 MiB scratch array that the
  * library holds on each device until the process ends.  Sizes with
- * n * (136 + 12 d) + 2^(d+2) + 2 MiB above 64 MiB are rejected with
- * SKY_ERR_CAPACITY (at d = 16 about 197,000 points, at d = 8 about 280,000).
+ * n * (138 + 12 d) + 2^(d+2) + 2 MiB above 64 MiB are rejected with
+ * SKY_ERR_CAPACITY (at d = 16 about 196,000 points, at d = 8 about 278,000).
  * The context supplies only its stream; its own buffers and held results are
  * left alone, and no result is held afterwards (sky_mdmc_fetch / _hashcube
  * report SKY_ERR_STATE).  All calls on one device share its scratch array:

@@ -423,18 +423,20 @@ __global__ void k_bgroup_counts(const uint32_t* __restrict__ eoff, uint32_t T,
 }
 
 // Group g (of all instances, goff[T] in total) of instance t: codes in code
-// order, candidate ids local to the instance, per-column minimum codes.
+// order, candidate ids local to the instance, per-column minimum codes of
+// the group (gsum[g]) and of its subgroups of 8 (ssum[8g ..]).
 template <int D>
 __global__ void k_bgroup_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ sval,
                              const uint32_t* __restrict__ eoff, uint32_t T,
                              const uint32_t* __restrict__ goff, uint2* __restrict__ gcode,
-                             uint32_t* __restrict__ glocal, uint2* __restrict__ gsum) {
+                             uint32_t* __restrict__ glocal, uint2* __restrict__ gsum,
+                             uint2* __restrict__ ssum) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= goff[T]) return;
   const uint32_t t = seg_of(goff, T, g);
   const uint32_t e0 = eoff[t], e1 = eoff[t + 1];
   const uint32_t j0 = e0 + (g - goff[t]) * 64;
-  group_sum<D>(codes, sval, j0, min(e1, j0 + 64), e0, gcode, glocal, gsum + g);
+  group_sum<D>(codes, sval, j0, min(e1, j0 + 64), e0, gcode, glocal, gsum + g, ssum + 8u * g);
 }
 
 // flag_pt[spid[j]] = 1 and flag_s1[j] = 1 for survivors nothing strictly
@@ -499,17 +501,26 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // Membership, persistent: each workgroup takes (instance, block) tasks from
 // a device counter until the group's tasks (toff[T]) are done.  With goff,
 // every instance visits its candidates in code-order groups (instance t's
-// groups from goff[t] on; gcode / glocal in the group code order); else with gfirst,
+// groups from goff[t] on, their subgroups from ssum + 8 goff[t] on; gcode /
+// glocal in the group code order); else with gfirst,
 // instance t's buckets are gfirst[t] .. gfirst[t+1]-1 of the bucket list
 // (bmk, bstart, bcnt over the bucket order bcode / blocal), used when they
 // hold min_bucket candidates or more on average; cover as in member_body
 // (0: auto_cover of each instance's extended skyline).
-// GROUPS = true compiles the code-order group visit only (goff must be set);
+// ORD = kOrderGroup or kOrderSub compiles that code-order visit only (goff
+// must be set); ORD = 0 picks buckets or the scan per instance;
 // with workgroups of four waves it also holds the kernel to 64 VGPRs, so that
 // eight waves run per SIMD (with a grid of 2048 workgroups); the few values
-// that then live in scratch cost less than the doubled occupancy gains.
-template <int D, int P, int NT, bool GROUPS = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUPS && NT == 256 ? 8 : 1)))
+// that then live in scratch cost less than the doubled occupancy gains
+// (bmember_waves; the subgroup steps with eight points per workgroup from
+// d = 8 on need more registers than that).
+template <int D, int P, int NT, int ORD>
+constexpr int bmember_waves() {
+  return ORD != 0 && NT == 256 && !(ORD == kOrderSub && P == 8 && D >= 8) ? 8 : 1;
+}
+
+template <int D, int P, int NT, int ORD = 0>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bmember_waves<D, P, NT, ORD>())))
 void k_bmember(const float* __restrict__ erows,
                                                 const uint2* __restrict__ codes,
                                                 const uint32_t* __restrict__ pos,
@@ -522,6 +533,7 @@ void k_bmember(const float* __restrict__ erows,
                                                 uint32_t head, uint32_t min_bucket,
                                                 const uint32_t* __restrict__ goff,
                                                 const uint2* __restrict__ gsum,
+                                                const uint2* __restrict__ ssum,
                                                 const uint2* __restrict__ gcode,
                                                 const uint32_t* __restrict__ glocal,
                                                 const uint32_t* __restrict__ gfirst,
@@ -541,6 +553,7 @@ void k_bmember(const float* __restrict__ erows,
     const uint32_t t = seg_of(toff, T, task);
     const uint32_t e0 = eoff[t], m = eoff[t + 1] - e0;
     uint32_t g0 = 0, nb = 0;
+    constexpr bool GROUPS = ORD != 0;
     if (gfirst && !GROUPS) {
       g0 = gfirst[t];
       nb = gfirst[t + 1] - g0;
@@ -549,9 +562,10 @@ void k_bmember(const float* __restrict__ erows,
     CandIndex ix;
     ix.head = head;
     int order = use_idx ? kOrderBucket : kOrderScan;
-    if (GROUPS || goff) {
-      order = kOrderGroup;
+    if (GROUPS) {
+      order = ORD;
       ix.gsum = gsum + goff[t];
+      ix.ssum = ssum + 8u * goff[t];
       ix.gcode = gcode + e0;
       ix.gidx = glocal + e0;
     } else if (use_idx) {

@@ -312,7 +312,7 @@ int skyi::batch_buffers(sky_ctx* c, size_t N, size_t T, uint64_t tab) {
   TRY(ensure(c, B.igfirst, sizeof(uint32_t) * (T + 1)));
   TRY(ensure(c, B.gcnt, sizeof(uint32_t) * (T + 1)));
   TRY(ensure(c, B.goff, sizeof(uint32_t) * (T + 1)));
-  TRY(ensure(c, B.gsum, sizeof(uint2) * (N / 64 + T + 1)));
+  TRY(ensure(c, B.gsum, sizeof(uint2) * 9 * gsum_cap(N, T)));  // groups, then subgroups
   TRY(ensure(c, B.iboff, sizeof(uint32_t) * tab));
   const size_t tb = prim_tmp_bytes(std::max<uint64_t>({N, T + 1, 1}));
   TRY(ensure(c, B.tmp, tb));
@@ -740,7 +740,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
   const int opt = option & 0xFF, dd = option >> 8;
   if (dd < 0 || dd > 16) return fail(c, SKY_ERR_ARG, "option %d: dimensionality %d", option, dd);
   if (opt == SKY_OPT_MEMBER_INDEX) {
-    if (value < -1 || value > 2 || dd) return fail(c, SKY_ERR_ARG, "bad member index option");
+    if (value < -1 || value > 3 || dd) return fail(c, SKY_ERR_ARG, "bad member index option");
     c->opt_member_index = (int)value;
     return SKY_OK;
   }
@@ -755,7 +755,7 @@ int sky_ctx_set_option(sky_ctx* c, int option, int64_t value) {
       {SKY_OPT_MIN_BUCKET, 1, 1 << 30, &Tuning::min_bucket},
       {SKY_OPT_MEMBER_GRID, 64, 1 << 13, &Tuning::grid},
       {SKY_OPT_TILE, 0, 1, &Tuning::tile},
-      {SKY_OPT_BATCH_ORDER, 0, 2, &Tuning::order},
+      {SKY_OPT_BATCH_ORDER, 0, 3, &Tuning::order},
       {SKY_OPT_COVER_HEAD, 0, 1 << 16, &Tuning::head},
   };
   for (const Range& r : ranges) {
@@ -806,7 +806,7 @@ int sky_batch_autotune(sky_ctx* c, const float* rows, int rows_on_device, uint32
     std::vector<uint32_t> values;
   };
   const Knob knobs[] = {
-      {"order", &Tuning::order, {2, 0}},
+      {"order", &Tuning::order, {3, 2, 0}},
       {"tile", &Tuning::tile, {0, 1}},
       {"cover", &Tuning::cover, {0, 512, 1024, 2048, 4096}},  // 0: auto_cover
       {"pruners", &Tuning::pruners, {128, 256, 512, 1024}},

@@ -57,7 +57,7 @@ struct BatchState {
   DevBuf ikey, ikey2, ival, isort, ibkey, ibcode, iblocal, ihead, ibmk, ibstart, ibcnt, ibrun,
       igfirst, iboff;
   // code-order groups: per-instance group counts / offsets, summaries
-  DevBuf gcnt, goff, gsum;
+  DevBuf gcnt, goff, gsum;  // gsum: gsum_cap group summaries, then 8 subgroup summaries each
   DevBuf sminkey;         // column minima of the survivors per instance
   DevBuf rs_key, rs_val;  // the other half of the radix sorts' ping-pong
   size_t tmp_bytes = 0;
@@ -70,7 +70,7 @@ struct Tuning {
   uint32_t min_bucket = sky::kMinBucket;  // bucket-index threshold
   uint32_t grid = 2048;                   // workgroups of the persistent kernel
   uint32_t tile = 0;                      // 0: member_points, 1: half of it
-  uint32_t order = 2;                     // batched membership: 2 code-order groups, 0 scan / buckets
+  uint32_t order = 3;                     // batched membership: 3 subgroups, 2 groups, 0 scan / buckets
   uint32_t head = 0;                      // cover head tested pair by pair (0: cover / 4)
 };
 
@@ -226,6 +226,9 @@ inline sky::DimList dims_of(uint32_t sub, uint32_t* ds) {
 
 // The batched survivor pass uses the mask index when its bucket table (one
 // entry per instance and median mask) stays within 2^26 entries.
+// Code-order groups of a batch group of N points in T instances, at most
+// (the subgroup summaries follow that many group summaries in gsum).
+inline size_t gsum_cap(uint32_t N, uint32_t T) { return (size_t)N / 64 + T + 1; }
 inline bool batch_self_index(int D, uint32_t T) { return ((uint64_t)T << D) <= (1ull << 26); }
 
 constexpr int kGroupCounters = 8;  // err, survivors, ext (id order), ext (score order), work

@@ -680,7 +680,18 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
 //                 (sorted by the bit-plane code key): gcode / gidx are the
 //                 codes and candidate ids in that order, gsum[g] the
 //                 per-column minimum code of group g as bit planes
-constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2;
+//   kOrderSub     the same groups, and ssum[8g + s] the minimum code of
+//                 subgroup s of group g (members 8s .. 8s+7): live groups
+//                 are narrowed to their live subgroups before any member
+//                 is tested
+constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2, kOrderSub = 3;
+// Default order of the single-instance path (measured: the subgroup steps
+// pay off up to d = 14; at d = 15, 16, with two points per workgroup, the
+// plain group rounds are faster).
+template <int D>
+constexpr int single_order() {
+  return D <= 14 ? kOrderSub : kOrderGroup;
+}
 struct CandIndex {
   uint32_t head = 0;  // cover candidates tested pair by pair (0: a quarter of the cover)
   const uint32_t* bmk = nullptr;
@@ -690,6 +701,7 @@ struct CandIndex {
   const uint32_t* bidx = nullptr;
   const uint2* bcode = nullptr;
   const uint2* gsum = nullptr;
+  const uint2* ssum = nullptr;  // subgroup g * 8 + s: members 8s .. 8s+7 of group g
   const uint2* gcode = nullptr;
   const uint32_t* gidx = nullptr;
   // the points: pts[0 .. npts-1] if set (else positions 0 .. m-1, through
@@ -780,11 +792,16 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __shared__ uint32_t U[P][NU];
   __shared__ float prow[P][D];
   __shared__ uint32_t queue[QCAP];
-  __shared__ uint32_t sidx[NT / 64][3][64];
+  // per-wave lists: the code-order groups' two lists (kList1 + kList2
+  // entries), or the bucket order's three arrays of 64
+  constexpr int kList1 = 72, kList2 = 64 + 64 * P;
+  constexpr int kWList = kList1 + kList2 > 192 ? kList1 + kList2 : 192;
+  __shared__ uint32_t wlist[NT / 64][kWList];
   __shared__ int s_any[P];
   __shared__ uint32_t s_zm[P];
   __shared__ uint32_t s_rgt[P];
   __shared__ uint32_t s_pt[P];  // the points: entry base + pp of the score order, the code order or ix.pts
+  __shared__ uint2 s_pc[P];     // their codes
   __shared__ uint32_t s_ok;     // bit pp: point pp exists and is computed here
   __shared__ int s_cnt;
   uint16_t* xl = reinterpret_cast<uint16_t*>(queue);  // X list reuses the queue
@@ -796,7 +813,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     // code order, so that the groups they need coincide
     const uint32_t np = ix.pts ? ix.npts : m;
     const uint32_t bt = min(base + tid, np - 1u);
-    const uint32_t a = ix.pts ? ix.pts[bt] : (order == kOrderGroup ? ix.gidx[bt] : bt);
+    const uint32_t a =
+        ix.pts ? ix.pts[bt] : (order == kOrderGroup || order == kOrderSub ? ix.gidx[bt] : bt);
     bool ok = base + tid < np;
     s_pt[tid] = a;
     uint32_t z = 0, g = 0;
@@ -817,6 +835,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     }
     const uint32_t okm = (uint32_t)__ballot(ok);  // the P lanes are in wave 0
     if (tid == 0) s_ok = okm;
+    s_pc[tid] = codes[a];
     s_zm[tid] = z;
     s_rgt[tid] = g;
     s_any[tid] = 0;
@@ -1020,6 +1039,122 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         }
       }
     }
+  } else if (order == kOrderSub) {
+    // Groups of 64 candidates of the code order, in three steps.  For q in
+    // group g, every code of q is >= the group's column minimum, so the
+    // columns where gsum[g] <= code(p) contain the code bound of every
+    // member: if that set is in A0 the whole group is skipped for p.  The
+    // same holds for each subgroup of 8 with its own minimum (ssum), which
+    // is tighter.  Each step feeds a per-wave list in LDS that the next
+    // step consumes with every lane busy:
+    //   1. lane l tests group g0 + l for the workgroup's points (neighbours
+    //      in the code order, so they mostly need the same groups); live
+    //      groups go to list 1 (group << 8 | points);
+    //   2. eight live groups at a time, lane l tests subgroup l & 7 of
+    //      group l >> 3; live (subgroup, point) pairs go to list 2;
+    //   3. 64 pairs at a time, lane l code-tests the eight members of its
+    //      subgroup (64 contiguous bytes of codes) against its point.
+    // The waves split the groups; a step runs when its list holds a full
+    // round (or, at the end, whatever is left).
+    constexpr int NW = NT / 64;
+    const int wv = tid >> 6;
+    const uint32_t mine = s_ok;
+    const uint32_t ng = (m + 63) / 64;
+    uint32_t* l1 = wlist[wv];            // < 72 entries
+    uint32_t* l2 = wlist[wv] + kList1;   // < 64 + 64 P entries: subgroup << 3 | point
+    uint32_t n1 = 0, n2 = 0;             // wave-uniform
+    auto sync_wave = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    };
+    uint32_t g0 = (uint32_t)wv * 64;
+    while (true) {  // uniform
+      const bool more = g0 < ng;
+      if (more) {  // step 1
+        const uint32_t g = g0 + (uint32_t)lane;
+        uint32_t al = 0;
+        if (g < ng) {
+          const uint2 gs = ix.gsum[g];
+#pragma unroll
+          for (int pp = 0; pp < P; ++pp) {
+            if (!((mine >> pp) & 1u)) continue;  // uniform
+            const uint32_t ub = planes_le(gs, pc[pp]) & FULL;
+            al |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << pp;
+          }
+        }
+        const uint64_t bl = __ballot(al != 0u);
+        if (al)
+          l1[n1 + __builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u))] =
+              (g << 8) | al;
+        n1 += (uint32_t)__popcll(bl);
+        g0 += (uint32_t)NW * 64;
+      }
+#pragma unroll 1
+      while (true) {  // uniform
+        if (n2 >= 64 || (!more && n1 == 0 && n2 > 0)) {  // step 3: the last 64 pairs
+          sync_wave();
+          const uint32_t take = min(n2, 64u);
+          const bool v = (uint32_t)lane < take;
+          const uint32_t ent = v ? l2[n2 - take + (uint32_t)lane] : 0u;
+          sync_wave();
+          n2 -= take;
+          const int pp = (int)(ent & 7u);
+          const uint32_t j0 = (ent >> 3) * 8u;  // < m for every pair
+          uint2 qc[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) qc[i] = ix.gcode[min(j0 + (uint32_t)i, m - 1u)];
+          const uint2 pcv = s_pc[pp];
+          uint32_t hit = 0;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const uint32_t ub = planes_le(qc[i], pcv) & FULL;
+            hit |= ((~(A0[pp][ub >> 5] >> (ub & 31))) & 1u) << i;
+          }
+          if (!v) hit = 0;
+          if (m - j0 < 8u) hit &= (1u << (m - j0)) - 1u;
+          if (__ballot(hit != 0u) == 0ull) continue;  // uniform
+#pragma unroll 1
+          for (int i = 0; i < 8; ++i) {
+            uint32_t h = (hit >> i) & 1u;
+            if (__ballot(h != 0u) == 0ull) continue;  // uniform
+            const uint32_t b = h ? ix.gidx[j0 + (uint32_t)i] : 0u;
+            if (b < kc) h = 0;  // the cover candidates were marked in pass 1a
+            append(h, 0, pp, b);
+            if (nq >= 64) drain();
+          }
+          continue;
+        }
+        if (n1 >= 8 || (!more && n1 > 0)) {  // step 2: the last eight live groups
+          sync_wave();
+          const uint32_t take = min(n1, 8u);
+          const uint32_t k = (uint32_t)lane >> 3;
+          const bool v = k < take;
+          const uint32_t ent = v ? l1[n1 - take + k] : 0u;
+          sync_wave();
+          n1 -= take;
+          const uint32_t gal = ent & 0xFFu;
+          const uint32_t sg = (ent >> 8) * 8u + (uint32_t)(lane & 7);
+          const bool sv = v && sg * 8u < m;  // the subgroup has members
+          const uint2 ss = ix.ssum[sv ? sg : 0u];
+#pragma unroll
+          for (int pp = 0; pp < P; ++pp) {
+            if (!((mine >> pp) & 1u)) continue;  // uniform
+            const uint32_t ub = planes_le(ss, pc[pp]) & FULL;
+            const bool lv = sv && ((gal >> pp) & 1u) && !((A0[pp][ub >> 5] >> (ub & 31)) & 1u);
+            const uint64_t bl = __ballot(lv);
+            if (lv)
+              l2[n2 + __builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32),
+                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u))] =
+                  (sg << 3) | (uint32_t)pp;
+            n2 += (uint32_t)__popcll(bl);
+          }
+          continue;
+        }
+        break;
+      }
+      if (!more) break;
+    }
   } else if (order == kOrderBucket) {
     // Candidates grouped by median mask (plane 3 of the codes): bucket g
     // holds the candidates of mask bmk[g] at positions bstart[g] ..
@@ -1030,9 +1165,9 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     // keep those alive for some point, and code-test their candidates 64
     // at a time (lane e finds its bucket by a binary search of the prefix
     // sums in LDS).
-    uint32_t* sLo = sidx[tid >> 6][0];
-    uint32_t* sEx = sidx[tid >> 6][1];
-    uint32_t* sAl = sidx[tid >> 6][2];
+    uint32_t* sLo = wlist[tid >> 6];
+    uint32_t* sEx = wlist[tid >> 6] + 64;
+    uint32_t* sAl = wlist[tid >> 6] + 128;
     uint32_t mp[P];
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) mp[pp] = pc[pp].x >> 16;
@@ -1245,7 +1380,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
           if (!(s & ~le) && (s & lt)) lds_mark(A1[pp], s);
         }
       };
-      if (order == kOrderGroup) {
+      if (order == kOrderGroup || order == kOrderSub) {
         // a group's code bound contains each member's, and the closure of X
         // in A0 is upward closed: a group whose bound is outside it holds
         // no candidate that passes the first filter
@@ -1330,41 +1465,61 @@ __global__ void k_plane_keys(const uint2* __restrict__ codes, uint32_t m,
 // Group of the code order at positions j0 .. j1-1: the codes in that order
 // (gcode[j] = codes[sidx[j]]), the candidate ids minus bias (gidx, if
 // given) and the per-column minimum code as bit planes (*gsum).
+// Bit planes of per-column codes (4 bits each).
 template <int D>
-__device__ __forceinline__ void group_sum(const uint2* __restrict__ codes,
-                                          const uint32_t* __restrict__ sidx, uint32_t j0,
-                                          uint32_t j1, uint32_t bias, uint2* __restrict__ gcode,
-                                          uint32_t* __restrict__ gidx, uint2* __restrict__ gsum) {
-  uint32_t mn[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) mn[k] = 15;
-  for (uint32_t j = j0; j < j1; ++j) {
-    const uint32_t i = sidx[j];
-    const uint2 c = codes[i];
-    gcode[j] = c;
-    if (gidx) gidx[j] = i - bias;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const uint32_t v = ((c.x >> (16 + k)) & 1u) << 3 | ((c.y >> (16 + k)) & 1u) << 2 |
-                         ((c.x >> k) & 1u) << 1 | ((c.y >> k) & 1u);
-      mn[k] = min(mn[k], v);
-    }
-  }
+__device__ __forceinline__ uint2 planes_of(const uint32_t* mn) {
   uint32_t pl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int k = 0; k < D; ++k)
 #pragma unroll
     for (int b = 0; b < 4; ++b) pl[b] |= ((mn[k] >> b) & 1u) << k;
-  *gsum = make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
+  return make_uint2((pl[3] << 16) | pl[1], (pl[2] << 16) | pl[0]);
 }
 
-// Codes in code order and each group's summary (one thread per group of 64).
+// ... and the same for each subgroup of 8 (ssum[0 .. 7]; a subgroup past j1
+// gets all-15 codes).
+template <int D>
+__device__ __forceinline__ void group_sum(const uint2* __restrict__ codes,
+                                          const uint32_t* __restrict__ sidx, uint32_t j0,
+                                          uint32_t j1, uint32_t bias, uint2* __restrict__ gcode,
+                                          uint32_t* __restrict__ gidx, uint2* __restrict__ gsum,
+                                          uint2* __restrict__ ssum) {
+  uint32_t mn[D], sm[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) mn[k] = 15;
+  for (int s = 0; s < 8; ++s) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) sm[k] = 15;
+    const uint32_t a = j0 + 8u * (uint32_t)s;
+    for (uint32_t j = a; j < min(j1, a + 8u); ++j) {
+      const uint32_t i = sidx[j];
+      const uint2 c = codes[i];
+      gcode[j] = c;
+      if (gidx) gidx[j] = i - bias;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const uint32_t v = ((c.x >> (16 + k)) & 1u) << 3 | ((c.y >> (16 + k)) & 1u) << 2 |
+                           ((c.x >> k) & 1u) << 1 | ((c.y >> k) & 1u);
+        sm[k] = min(sm[k], v);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) mn[k] = min(mn[k], sm[k]);
+    ssum[s] = planes_of<D>(sm);
+  }
+  *gsum = planes_of<D>(mn);
+}
+
+// Codes in code order and the summaries of each group of 64 (one thread per
+// group): gsum[g], and its subgroups' at gsum[ng + 8g ..].
 template <int D>
 __global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __restrict__ gidx,
                             uint32_t m, uint2* __restrict__ gcode, uint2* __restrict__ gsum) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (m + 63) / 64) return;
-  group_sum<D>(codes, gidx, g * 64, min(m, g * 64 + 64), 0u, gcode, nullptr, gsum + g);
+  const uint32_t ng = (m + 63) / 64;
+  if (g >= ng) return;
+  group_sum<D>(codes, gidx, g * 64, min(m, g * 64 + 64), 0u, gcode, nullptr, gsum + g,
+               gsum + ng + 8u * g);
 }
 
 // HashCube export: (word index, word value) -> point ids.  The bitmaps are

@@ -44,12 +44,13 @@ int noalloc_init_device(int device) {
 uint64_t noalloc_sort_tmp(uint64_t n) { return 16ull * n + (1ull << 20); }
 
 uint64_t noalloc_bytes(uint64_t n, uint32_t d) {
-  // every buffer of the call with n_ext = n: 120 + 12 d bytes per point (the
+  // every buffer of the call with n_ext = n: 122 + 12 d bytes per point (the
   // score-order keys, ids, rows and codes of the split, its mask index, the
-  // extended skyline's rows, ids and code order, the deferral list), the
+  // extended skyline's rows, ids and code order with its group and subgroup
+  // summaries, the deferral list), the
   // rocPRIM temporaries, the median-mask bucket table of 2^d + 1 entries, and
   // 1 MiB for the 256-byte rounding of about 50 carves and the small tables
-  return n * (120ull + 12ull * d) + noalloc_sort_tmp(n) + (4ull << d) + (1ull << 20);
+  return n * (122ull + 12ull * d) + noalloc_sort_tmp(n) + (4ull << d) + (1ull << 20);
 }
 
 }  // namespace skyi

@@ -51,23 +51,27 @@ int batch_member(sky_ctx* c, BatchGroup& g, const uint32_t* eoff, const Tuning& 
   LAUNCHCHK();
   TRY(exclusive_scan_dev(c, P<uint32_t>(B.seg_a), P<uint32_t>(B.seg_b), T + 1));
   constexpr int NT = sky::member_threads<D>();
-  if (tu.order == 2)  // code-order groups for every instance
-    hipLaunchKernelGGL((sky::k_bmember<D, PP, NT, true>), dim3(tu.grid), dim3(NT), 0, c->stream,
+  if (tu.order == 2 || tu.order == 3) {  // code-order groups (or subgroups) for every instance
+    auto kern = tu.order == 3 ? sky::k_bmember<D, PP, NT, sky::kOrderSub>
+                              : sky::k_bmember<D, PP, NT, sky::kOrderGroup>;
+    hipLaunchKernelGGL(kern, dim3(tu.grid), dim3(NT), 0, c->stream,
                        P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                        P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
                        P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head,
-                       tu.min_bucket, P<uint32_t>(B.goff), P<uint2>(B.gsum), P<uint2>(B.ibcode),
+                       tu.min_bucket, P<uint32_t>(B.goff), P<uint2>(B.gsum),
+                       P<uint2>(B.gsum) + gsum_cap(g.N, T), P<uint2>(B.ibcode),
                        P<uint32_t>(B.iblocal), nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr);
-  else
+  } else {
     hipLaunchKernelGGL((sky::k_bmember<D, PP, NT>), dim3(tu.grid), dim3(NT), 0, c->stream,
                        P<float>(B.erows), P<uint2>(B.codes), P<uint32_t>(B.pos), eoff, T,
                        P<uint32_t>(B.seg_b), P<uint32_t>(g.d_maxd), P<uint32_t>(B.minkey),
                        P<uint32_t>(g.dom), cnt + 4, tu.cover, tu.head,
-                       tu.min_bucket, nullptr, P<uint2>(B.gsum), P<uint2>(B.ibcode),
+                       tu.min_bucket, nullptr, P<uint2>(B.gsum), nullptr, P<uint2>(B.ibcode),
                        P<uint32_t>(B.iblocal), P<uint32_t>(B.igfirst), P<uint32_t>(B.ibmk),
                        P<uint32_t>(B.ibstart), P<uint32_t>(B.ibcnt), P<uint32_t>(B.iblocal),
                        P<uint2>(B.ibcode));
+  }
   LAUNCHCHK();
   return SKY_OK;
 }
@@ -174,7 +178,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
   hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream, P<float>(B.erows),
                      eoff, T, cnt + 3, P<uint32_t>(B.piv), P<uint2>(B.codes));
   LAUNCHCHK();
-  if (tu.order == 2) {
+  if (tu.order == 2 || tu.order == 3) {
     // code order per instance, groups of 64 with their summaries: one sort
     // of (instance, code key) -- the instance in the top bits, the lowest
     // code-key bits dropped where both do not fit 64 bits (any order is
@@ -194,7 +198,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
     hipLaunchKernelGGL(sky::k_bgroup_sum<D>, dim3(grid_for(N / 64 + T + 1)), dim3(kBlock), 0,
                        c->stream, P<uint2>(B.codes), P<uint32_t>(B.isort), eoff, T,
                        P<uint32_t>(B.goff), P<uint2>(B.ibcode), P<uint32_t>(B.iblocal),
-                       P<uint2>(B.gsum));
+                       P<uint2>(B.gsum), P<uint2>(B.gsum) + gsum_cap(N, T));
     LAUNCHCHK();
   } else {
     // bucket list of the extended skyline per instance (count in cnt[5])

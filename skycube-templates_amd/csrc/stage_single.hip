@@ -79,7 +79,7 @@ int member_index(sky_ctx* c, uint32_t d, uint32_t ne, uint32_t* nbk) {
 
 // Code order of the ne extended-skyline points for kOrderGroup: bidx =
 // positions in order of the bit-plane code key, bcode = their codes, gsum =
-// per-column minimum code of every group of 64.
+// per-column minimum code of every group of 64, then of every subgroup of 8.
 template <int D>
 int member_groups(sky_ctx* c, uint32_t ne) {
   const uint32_t ng = (ne + 63) / 64;
@@ -88,7 +88,7 @@ int member_groups(sky_ctx* c, uint32_t ne) {
   TRY(ensure(c, c->biota, sizeof(uint32_t) * ne));
   TRY(ensure(c, c->bidx, sizeof(uint32_t) * ne));
   TRY(ensure(c, c->bcode, sizeof(uint2) * ne));
-  TRY(ensure(c, c->gsum, sizeof(uint2) * ng));
+  TRY(ensure(c, c->gsum, sizeof(uint2) * 9 * ng));  // groups, then their subgroups
   hipLaunchKernelGGL(sky::k_plane_keys<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
                      P<uint2>(c->codes), ne, P<uint64_t>(c->hc_key_a), P<uint32_t>(c->biota));
   LAUNCHCHK();
@@ -238,7 +238,7 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
   constexpr int NT = sky::member_threads<D>();
   const Tuning& tu = c->tune[D];
   int order = c->opt_member_index;
-  if (order < 0) order = sky::kOrderGroup;
+  if (order < 0) order = sky::single_order<D>();
   const uint32_t cover = tu.cover ? tu.cover : sky::auto_cover(ne);
   sky::CandIndex ix;
   ix.head = tu.head;
@@ -249,12 +249,15 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
     ix.nbk = nbk;
     ix.bidx = P<uint32_t>(c->bidx);
     ix.bcode = P<uint2>(c->bcode);
-  } else if (order == sky::kOrderGroup) {
+  } else if (order == sky::kOrderGroup || order == sky::kOrderSub) {
     TRY(member_groups<D>(c, ne));
     ix.gsum = P<uint2>(c->gsum);
+    ix.ssum = P<uint2>(c->gsum) + (ne + 63) / 64;
     ix.gcode = P<uint2>(c->bcode);
     ix.gidx = P<uint32_t>(c->bidx);
   }
+  if (order == sky::kOrderSub)
+    return launch_member<D, PP, NT, sky::kOrderSub>(c, ne, max_d, minkey, cover, ix);
   if (order == sky::kOrderGroup)
     return launch_member<D, PP, NT, sky::kOrderGroup>(c, ne, max_d, minkey, cover, ix);
   if (order == sky::kOrderBucket)

@@ -63,7 +63,7 @@ def test_batch_options_do_not_change_results(ctx):
     settings = [(skycube_amd.SKY_OPT_COVER, 64), (skycube_amd.SKY_OPT_PRUNERS, 64),
                 (skycube_amd.SKY_OPT_MIN_BUCKET, 1), (skycube_amd.SKY_OPT_MEMBER_GRID, 64),
                 (skycube_amd.SKY_OPT_TILE, 1)]
-    for order in (0, 2):  # buckets (min_bucket 1), code-order groups
+    for order in (0, 2, 3):  # buckets (min_bucket 1), code-order groups, subgroups
         with skycube_amd.Context(0) as c:
             c.set_option(skycube_amd.SKY_OPT_BATCH_ORDER, order)
             for opt, val in settings:
@@ -134,7 +134,7 @@ def test_batch_wide_and_narrow_d(ctx):
         oe, od = skyorc.mdmc(mats[k], caps[k] or mats[k].shape[1])
         assert np.array_equal(alone[k][0], oe) and np.array_equal(alone[k][1], od), k
     for tile in (0, 1):
-        for order in (0, 2):
+        for order in (0, 2, 3):
             with skycube_amd.Context(0) as c:
                 c.set_option(skycube_amd.SKY_OPT_TILE, tile)
                 c.set_option(skycube_amd.SKY_OPT_BATCH_ORDER, order)

@@ -185,12 +185,12 @@ def test_hashcube_export(ctx, d, max_d, n):
                               np.sort(skycube_amd.members(dom, ext, s)))
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("d", [3, 6, 9, 12, 14, 16])
 def test_mdmc_member_candidate_order(d, mode):
     """Every membership candidate order (score-order scan, median-mask
-    buckets, code-order groups) gives the oracle's bitmaps, with ties and
-    without."""
+    buckets, code-order groups, groups narrowed to subgroups) gives the
+    oracle's bitmaps, with ties and without."""
     rng = np.random.default_rng(2000 + 17 * d + mode)
     n = 3000 if d <= 12 else 1200
     with skycube_amd.Context(0) as c:
