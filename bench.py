@@ -478,7 +478,8 @@ def main() -> int:
                    "hashcube": {"groups": win["groups"], "entries": win["entries"]}},
         "window_parts_ms": win["parts_ms"],
         "stages_ms": dict(st, device_total=round(dev_ms, 3)),
-        "roofline": {"bound": "hbm", "kernel": f"k_member<d={d}> (code-order groups)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_member<d={d}> (code-order {'groups' if d >= 15 else 'subgroups'})",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": B},

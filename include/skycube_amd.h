@@ -119,8 +119,9 @@ int sky_ctx_release(sky_ctx *ctx);
 #define SKY_STAGE_SCORE_SORT 0 /* Manhattan-norm keys, radix sort, gather */
 #define SKY_STAGE_PRUNE 1      /* prefilter against the lowest-score points */
 #define SKY_STAGE_SELF 2       /* survivors against each other */
-#define SKY_STAGE_COMPACT 3    /* extended skyline in id order, row gather */
-#define SKY_STAGE_MEMBER 4     /* per-point membership bitmaps */
+#define SKY_STAGE_COMPACT 3    /* extended skyline in id order, row gather,
+                                  column minima, codes, candidate order */
+#define SKY_STAGE_MEMBER 4     /* the membership kernel launches alone */
 #define SKY_STAGE_COUNT 5
 int sky_ctx_set_timing(sky_ctx *ctx, int enable);
 int sky_ctx_stage_times(const sky_ctx *ctx, float *ms, int count);

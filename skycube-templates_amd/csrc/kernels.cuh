@@ -1419,6 +1419,10 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
 
+// Tiles are taken in block order: the workgroups in flight then hold one
+// narrow window of the code order, spread over the eight XCDs.  (Giving each
+// XCD a contiguous run of tiles instead was measured slower: the cost of a
+// tile varies along the code order, so the XCDs finish unevenly.)
 template <int D, int P, int NT, int ORDER, int ZB = zbits<D>()>
 __global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
                                                const uint2* __restrict__ codes,

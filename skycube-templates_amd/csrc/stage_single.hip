@@ -231,7 +231,6 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
                      P<float>(c->erows), ne, P<uint32_t>(c->piv), P<uint2>(c->codes),
                      P<uint32_t>(c->masks));
   LAUNCHCHK();
-  TRY(mark(c, SKY_STAGE_MEMBER));
   uint32_t nbk = 0;
   if (c->opt_member_index == sky::kOrderBucket) TRY(member_index(c, D, ne, &nbk));
   constexpr int PP = sky::member_points<D>();
@@ -256,6 +255,7 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
     ix.gcode = P<uint2>(c->bcode);
     ix.gidx = P<uint32_t>(c->bidx);
   }
+  TRY(mark(c, SKY_STAGE_MEMBER));  // the membership kernel launches alone
   if (order == sky::kOrderSub)
     return launch_member<D, PP, NT, sky::kOrderSub>(c, ne, max_d, minkey, cover, ix);
   if (order == sky::kOrderGroup)
