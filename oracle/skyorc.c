@@ -405,48 +405,74 @@ int orc_ext_skyline_timed(const float *rows, uint32_t n, int d, uint32_t stride,
  * D = closure(A0)  union over j of ({s : j in s} intersect closure(plane j)).
  * Bits of subspaces outside the scope (bit 0, popcount > max_d) are set.
  */
-static void membership_one(const float *rows, int d, const uint32_t *ext,
-                           uint32_t n_ext, uint32_t pi, int max_d,
-                           uint32_t *out, uint32_t *planes) {
+#define ORC_BLOCK 32 /* points per block of membership_block */
+
+/* Membership of np <= ORC_BLOCK points (rows pts[b]) against the candidates
+ * cand[0 .. n_cand-1] (n_cand x d rows, contiguous).  The candidates are the
+ * outer loop, so each candidate row is read once per block rather than once
+ * per point.  out: np x W words; planes: np x d x W words of scratch. */
+static void membership_block(const float *rows, int d, const float *cand,
+                             uint32_t n_cand, const uint32_t *pts, int np,
+                             int max_d, uint32_t *out, uint32_t *planes) {
   const int W = orc_words_per_point(d);
-  const float *p = rows + (uint64_t)pi * d;
-  uint32_t used_planes = 0;
-  memset(out, 0, sizeof(uint32_t) * W);
-  for (uint32_t k = 0; k < n_ext; ++k) {
-    const float *q = rows + (uint64_t)ext[k] * d;
-    uint32_t lt, eq;
-    pair_masks(q, p, d, &lt, &eq);
-    const uint32_t le = lt | eq;
-    if (lt == 0)
-      continue;
-    if (eq == 0) {
-      set_bit(out, le);
-    } else {
-      for (uint32_t t = lt; t; t &= t - 1) {
-        const int j = __builtin_ctz(t);
-        if (!(used_planes >> j & 1u)) {
-          memset(planes + (size_t)j * W, 0, sizeof(uint32_t) * W);
-          used_planes |= 1u << j;
+  uint32_t used[ORC_BLOCK];
+  memset(out, 0, sizeof(uint32_t) * W * (size_t)np);
+  for (int b = 0; b < np; ++b)
+    used[b] = 0;
+  for (uint32_t k = 0; k < n_cand; ++k) {
+    const float *q = cand + (uint64_t)k * d;
+    for (int b = 0; b < np; ++b) {
+      const float *p = rows + (uint64_t)pts[b] * d;
+      uint32_t lt, eq;
+      pair_masks(q, p, d, &lt, &eq);
+      const uint32_t le = lt | eq;
+      if (lt == 0)
+        continue;
+      uint32_t *ob = out + (size_t)b * W;
+      if (eq == 0) {
+        set_bit(ob, le);
+      } else {
+        uint32_t *pb = planes + (size_t)b * d * W;
+        for (uint32_t t = lt; t; t &= t - 1) {
+          const int j = __builtin_ctz(t);
+          if (!(used[b] >> j & 1u)) {
+            memset(pb + (size_t)j * W, 0, sizeof(uint32_t) * W);
+            used[b] |= 1u << j;
+          }
+          set_bit(pb + (size_t)j * W, le);
         }
-        set_bit(planes + (size_t)j * W, le);
       }
     }
   }
-  down_closure(out, d);
-  for (uint32_t t = used_planes; t; t &= t - 1) {
-    const int j = __builtin_ctz(t);
-    uint32_t *pl = planes + (size_t)j * W;
-    down_closure(pl, d);
-    for (int k = 0; k < W; ++k) {
-      uint32_t m;
-      if (j < 5)
-        m = kHiHalf[j];
-      else
-        m = (k >> (j - 5) & 1) ? 0xFFFFFFFFu : 0u;
-      out[k] |= pl[k] & m;
+  for (int b = 0; b < np; ++b) {
+    uint32_t *ob = out + (size_t)b * W;
+    uint32_t *pb = planes + (size_t)b * d * W;
+    down_closure(ob, d);
+    for (uint32_t t = used[b]; t; t &= t - 1) {
+      const int j = __builtin_ctz(t);
+      uint32_t *pl = pb + (size_t)j * W;
+      down_closure(pl, d);
+      for (int k = 0; k < W; ++k) {
+        uint32_t m;
+        if (j < 5)
+          m = kHiHalf[j];
+        else
+          m = (k >> (j - 5) & 1) ? 0xFFFFFFFFu : 0u;
+        ob[k] |= pl[k] & m;
+      }
     }
+    mask_scope(ob, d, max_d);
   }
-  mask_scope(out, d, max_d);
+}
+
+/* The candidate rows (the extended skyline) gathered contiguously. */
+static float *gather_rows(const float *rows, int d, const uint32_t *ext, uint32_t m) {
+  float *c = (float *)malloc(sizeof(float) * (size_t)d * (m ? m : 1));
+  if (!c)
+    return NULL;
+  for (uint32_t k = 0; k < m; ++k)
+    memcpy(c + (size_t)k * d, rows + (uint64_t)ext[k] * d, sizeof(float) * d);
+  return c;
 }
 
 /*
@@ -466,22 +492,27 @@ int orc_mdmc(const float *rows, uint32_t n, int d, int max_d, uint32_t *ext_idx,
     return rc;
   const int W = orc_words_per_point(d);
   const uint32_t m = *n_ext;
+  float *cand = gather_rows(rows, d, ext_idx, m);
+  if (!cand)
+    return ORC_ERR_OOM;
   int oom = 0;
 #pragma omp parallel
   {
-    uint32_t *planes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * d);
+    uint32_t *planes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * d * ORC_BLOCK);
     if (!planes) {
 #pragma omp atomic write
       oom = 1;
     }
-#pragma omp for schedule(dynamic, 16)
-    for (uint32_t a = 0; a < m; ++a) {
+#pragma omp for schedule(dynamic, 1)
+    for (uint32_t a = 0; a < m; a += ORC_BLOCK) {
+      const int np = (int)(m - a < ORC_BLOCK ? m - a : ORC_BLOCK);
       if (planes)
-        membership_one(rows, d, ext_idx, m, ext_idx[a], max_d,
-                       dom_words + (size_t)a * W, planes);
+        membership_block(rows, d, cand, m, ext_idx + a, np, max_d,
+                         dom_words + (size_t)a * W, planes);
     }
     free(planes);
   }
+  free(cand);
   return oom ? ORC_ERR_OOM : ORC_OK;
 }
 
@@ -497,23 +528,34 @@ int orc_mdmc_sample(const float *rows, int d, int max_d, const uint32_t *ext_idx
   if (max_d > d)
     max_d = d;
   const int W = orc_words_per_point(d);
+  for (uint32_t k = 0; k < n_sample; ++k)
+    if (sample[k] >= n_ext)
+      return ORC_ERR_ARG;
+  float *cand = gather_rows(rows, d, ext_idx, n_ext);
+  if (!cand)
+    return ORC_ERR_OOM;
   int bad = 0;
 #pragma omp parallel
   {
-    uint32_t *planes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * d);
-#pragma omp for schedule(dynamic, 4)
-    for (uint32_t k = 0; k < n_sample; ++k) {
-      if (!planes || sample[k] >= n_ext) {
+    uint32_t *planes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)W * d * ORC_BLOCK);
+    uint32_t pts[ORC_BLOCK];
+#pragma omp for schedule(dynamic, 1)
+    for (uint32_t k0 = 0; k0 < n_sample; k0 += ORC_BLOCK) {
+      const int np = (int)(n_sample - k0 < ORC_BLOCK ? n_sample - k0 : ORC_BLOCK);
+      if (!planes) {
 #pragma omp atomic write
         bad = 1;
         continue;
       }
-      membership_one(rows, d, ext_idx, n_ext, ext_idx[sample[k]], max_d,
-                     dom_words + (size_t)k * W, planes);
+      for (int b = 0; b < np; ++b)
+        pts[b] = ext_idx[sample[k0 + b]];
+      membership_block(rows, d, cand, n_ext, pts, np, max_d, dom_words + (size_t)k0 * W,
+                       planes);
     }
     free(planes);
   }
-  return bad ? ORC_ERR_ARG : ORC_OK;
+  free(cand);
+  return bad ? ORC_ERR_OOM : ORC_OK;
 }
 
 /* ----------------------------------------------------------------------- */
