@@ -67,6 +67,14 @@ uint32_t sky_words_per_point(uint32_t d);
  * demand and are kept for the next call. */
 size_t sky_ctx_bytes_in_use(const sky_ctx *ctx);
 
+/* Page-locked host memory for rows and results (NULL if none is left):
+ * copies between it and the device run asynchronously at full link speed,
+ * where pageable memory is staged by the runtime.  The reference pins its
+ * input before the upload (src/mdmc/MDMC.cu:1068-1071); the skycube driver
+ * reads the rows into such a buffer.  Free with sky_host_free. */
+void *sky_host_alloc(size_t bytes);
+void sky_host_free(void *ptr);
+
 /* Tuning options of a context (results never depend on them).  No
  * reference counterpart (the reference's kernels have fixed launch shapes).
  *   SKY_OPT_MEMBER_INDEX  how the membership kernel of the single-instance

@@ -879,6 +879,16 @@ const char* sky_last_error(const sky_ctx* c) { return c ? c->err.c_str() : "null
 
 size_t sky_ctx_bytes_in_use(const sky_ctx* c) { return c ? c->in_use : 0; }
 
+void* sky_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void sky_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int sky_ctx_set_timing(sky_ctx* c, int enable) {
   if (!c) return SKY_ERR_ARG;
   c->timing = enable != 0;

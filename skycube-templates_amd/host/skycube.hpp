@@ -10,6 +10,7 @@
 // All device work happens in libskycube_amd.so; errors raise SkyError.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -48,6 +49,52 @@ struct lattice_node {
 };
 
 using HashCubeMaps = std::map<unsigned int, std::vector<int>>;
+
+// Host buffer in page-locked memory (sky_host_alloc), or in ordinary memory
+// when none is left: copies between it and the device then run at full
+// link speed.
+template <typename T>
+class HostBuffer {
+ public:
+  HostBuffer() = default;
+  ~HostBuffer() { release(); }
+  HostBuffer(const HostBuffer&) = delete;
+  HostBuffer& operator=(const HostBuffer&) = delete;
+  // n elements; the contents are not kept when the buffer grows
+  void resize(size_t n) {
+    if (n > cap_) {
+      release();
+      const size_t bytes = sizeof(T) * (n ? n : 1);
+      void* p = sky_host_alloc(bytes);
+      pinned_ = p != nullptr;
+      if (!p) p = std::malloc(bytes);
+      if (!p) throw std::bad_alloc();
+      p_ = static_cast<T*>(p);
+      cap_ = n;
+    }
+    n_ = n;
+  }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  size_t size() const { return n_; }
+  T* begin() { return p_; }
+  T* end() { return p_ + n_; }
+  T& operator[](size_t i) { return p_[i]; }
+  bool pinned() const { return pinned_; }
+
+ private:
+  void release() {
+    if (p_) {
+      if (pinned_) sky_host_free(p_);
+      else std::free(p_);
+    }
+    p_ = nullptr;
+    cap_ = n_ = 0;
+  }
+  T* p_ = nullptr;
+  size_t n_ = 0, cap_ = 0;
+  bool pinned_ = false;
+};
 
 // Splits one CSV line on ',' (a trailing comma adds no field) and converts
 // every field with strtof, correctly rounded like istringstream >> float.
@@ -119,13 +166,17 @@ inline int first_device(const Config& cfg) { return cfg.devices.empty() ? 0 : cf
 class MDMC {
  public:
   MDMC(const Config& cfg, uint32_t n, uint32_t d) : cfg_(cfg), n_(n), d_(d), dev_(first_device(cfg)) {}
-  // Copies the rows (the reference copies them into a flat buffer, MDMC.cu:204-225).
+  // Copies the rows into pinned memory (the reference copies them into a
+  // flat buffer, MDMC.cu:204-225, and pins it, MDMC.cu:1068-1071).
   void Init(float** data) {
     rows_.resize((size_t)n_ * d_);
     for (uint32_t i = 0; i < n_; ++i)
       for (uint32_t j = 0; j < d_; ++j) rows_[(size_t)i * d_ + j] = data[i][j];
   }
-  void Init(const std::vector<float>& rows) { rows_ = rows; }
+  void Init(const std::vector<float>& rows) {
+    rows_.resize(rows.size());
+    std::copy(rows.begin(), rows.end(), rows_.begin());
+  }
   // Device skycube, then the HashCube grouped on the device and loaded
   // into the reference's map layout.
   void Execute() {
@@ -136,15 +187,19 @@ class MDMC {
     ext_.resize(ne);
     uint64_t ng = 0, nt = 0;
     dev_.check(sky_mdmc_hashcube(dev_.get(), &ng, &nt, nullptr, nullptr, nullptr, nullptr, 0, 0));
-    std::vector<uint32_t> gw(ng ? ng : 1), gv(ng ? ng : 1), ids(nt ? nt : 1);
+    std::vector<uint32_t> gw(ng ? ng : 1), gv(ng ? ng : 1);
     std::vector<uint64_t> go(ng + 1);
+    ids_.resize(nt ? nt : 1);
+    HostBuffer<uint32_t>& ids = ids_;
     dev_.check(sky_mdmc_hashcube(dev_.get(), &ng, &nt, gw.data(), gv.data(), go.data(), ids.data(),
                                  gw.size(), ids.size()));
     const uint32_t words = sky_words_per_point(d_) + (d_ >= 5 ? 1u : 0u);
     maps_.assign(words, HashCubeMaps());
+    // the groups come sorted by (word, value): each insertion goes at the
+    // end of its map
     for (uint64_t g = 0; g < ng; ++g) {
-      std::vector<int>& v = maps_[gw[g]][gv[g]];
-      v.assign(ids.begin() + (long)go[g], ids.begin() + (long)go[g + 1]);
+      HashCubeMaps& mp = maps_[gw[g]];
+      mp.emplace_hint(mp.end(), gv[g], std::vector<int>(ids.begin() + go[g], ids.begin() + go[g + 1]));
     }
   }
   // (2^d / 32) + 1 maps, word value -> point ids (Hashcube.h:171-177).
@@ -156,7 +211,8 @@ class MDMC {
   Config cfg_;
   uint32_t n_, d_;
   Device dev_;
-  std::vector<float> rows_;
+  HostBuffer<float> rows_;
+  HostBuffer<uint32_t> ids_;  // the HashCube's ids as they come from the device
   std::vector<uint32_t> ext_;
   std::vector<HashCubeMaps> maps_;
 };
@@ -170,7 +226,10 @@ class SDSC {
     for (uint32_t i = 0; i < n_; ++i)
       for (uint32_t j = 0; j < d_; ++j) rows_[(size_t)i * d_ + j] = data[i][j];
   }
-  void Init(const std::vector<float>& rows) { rows_ = rows; }
+  void Init(const std::vector<float>& rows) {
+    rows_.resize(rows.size());
+    std::copy(rows.begin(), rows.end(), rows_.begin());
+  }
   void Execute() {
     const uint32_t max_d = cfg_.max_d > 0 ? (uint32_t)cfg_.max_d : d_;
     const uint32_t S = 1u << d_;
@@ -197,7 +256,7 @@ class SDSC {
   Config cfg_;
   uint32_t n_, d_;
   Device dev_;
-  std::vector<float> rows_;
+  HostBuffer<float> rows_;
   std::vector<lattice_node> lattice_;
 };
 

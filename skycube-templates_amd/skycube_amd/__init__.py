@@ -48,7 +48,7 @@ EXPORTS = (
     "sky_ctx_release", "sky_ctx_set_option", "sky_server_create", "sky_server_destroy", "sky_server_footprint",
     "sky_server_submit", "sky_server_wait", "sky_server_result", "sky_server_stats",
     "sky_mdmc_device_noalloc", "sky_sdsc_device", "sky_sdsc_fetch", "sky_batch_mdmc_dev",
-    "sky_ctx_stream",
+    "sky_ctx_stream", "sky_host_alloc", "sky_host_free",
 )
 STAGES = ("score_sort", "prune", "self", "compact", "member")
 

@@ -307,3 +307,34 @@ def test_sdsc_anticorrelated_full_is_golden(ctx):
         dom[pos[ids], s >> 5] &= np.uint32(~(1 << (s & 31)) & 0xFFFFFFFF)
     h = hashlib.sha256(pts.astype("<u4").tobytes() + dom.astype("<u4").tobytes()).hexdigest()
     assert h == gold["sha256"]
+
+
+def test_pinned_host_buffers(ctx):
+    """Rows read from, and ids written to, page-locked memory from
+    sky_host_alloc give the oracle's result, and the buffers free cleanly."""
+    import ctypes
+    L = skycube_amd.load()
+    L.sky_host_alloc.restype = ctypes.c_void_p
+    L.sky_host_alloc.argtypes = [ctypes.c_size_t]
+    L.sky_host_free.argtypes = [ctypes.c_void_p]
+    L.sky_host_free.restype = None
+    rng = np.random.default_rng(77)
+    x = npref.tie_heavy(rng, 4000, 10, 6)
+    n, d = x.shape
+    prow = L.sky_host_alloc(x.nbytes)
+    pext = L.sky_host_alloc(4 * n)
+    assert prow and pext
+    try:
+        rows = np.ctypeslib.as_array((ctypes.c_float * x.size).from_address(prow)).reshape(n, d)
+        rows[:] = x
+        ext_buf = np.ctypeslib.as_array((ctypes.c_uint32 * n).from_address(pext))
+        _say("pinned host rows and ids")
+        ne = ctx.mdmc_raw(prow, n, d, 7, pext)
+        ext = ext_buf[:ne].copy()
+        _, dom = ctx.mdmc_fetch(ne, d)
+        oext, odom = skyorc.mdmc(x, 7)
+        assert np.array_equal(ext, oext) and np.array_equal(dom, odom)
+    finally:
+        L.sky_host_free(prow)
+        L.sky_host_free(pext)
+    L.sky_host_free(None)  # a null pointer is ignored
