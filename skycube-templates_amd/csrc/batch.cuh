@@ -153,14 +153,16 @@ __global__ void k_bgather_idx(const float* __restrict__ srows, const uint32_t* _
   opid[j] = spid[i];
 }
 
-// Quantile pivots per instance and column (block = (t, column)); a strided
-// sample of up to 256 of the instance's rows is ranked by counting.
-// Instances without rows get pivots above every value.
+// Quantile pivots per instance and column (block = (t, column)): a strided
+// sample of up to 2048 of the instance's rows (as many as the
+// single-instance path takes), bitonic-sorted in LDS; the values of ranks
+// (b+1) * S / 16 become the pivots.  Instances without rows get pivots above
+// every value.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_bpivots(const float* __restrict__ rows,
                                                     const uint32_t* __restrict__ off, uint32_t T,
                                                     uint32_t* __restrict__ piv) {
-  constexpr int S = 256;
+  constexpr uint32_t S = 2048;
   __shared__ uint32_t val[S];
   const uint32_t t = blockIdx.x / D;
   const int col = blockIdx.x % D;
@@ -170,18 +172,28 @@ __global__ __launch_bounds__(kBlock) void k_bpivots(const float* __restrict__ ro
     if (threadIdx.x < kCodePivots) out[threadIdx.x] = 0xFFFFFFFFu;
     return;
   }
-  const uint32_t cnt = min((uint32_t)S, m);
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
-    val[i] = order_key(rows[(size_t)(b0 + (uint64_t)i * m / cnt) * D + col]);
+  const uint32_t cnt = min(S, m);
+  uint32_t p2 = 1;
+  while (p2 < cnt) p2 <<= 1;
+  for (uint32_t i = threadIdx.x; i < p2; i += kBlock)
+    val[i] = i < cnt ? order_key(rows[(size_t)(b0 + (uint64_t)i * m / cnt) * D + col]) : 0xFFFFFFFFu;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-    const uint32_t v = val[i];
-    uint32_t r = 0;
-    for (uint32_t j = 0; j < cnt; ++j) r += (val[j] < v) || (val[j] == v && j < i);
-#pragma unroll 1
-    for (int b = 0; b < kCodePivots; ++b)
-      if ((uint32_t)((b + 1) * cnt / (kCodePivots + 1)) == r) out[b] = v;
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < p2; i += kBlock) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint32_t a = val[i], b = val[l];
+          if ((a > b) == ((i & k) == 0)) {
+            val[i] = b;
+            val[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
   }
+  if (threadIdx.x < kCodePivots) out[threadIdx.x] = val[(threadIdx.x + 1) * cnt / (kCodePivots + 1)];
 }
 
 template <int D>
@@ -463,7 +475,10 @@ __global__ void k_bpos(const uint32_t* __restrict__ epid, const uint32_t* __rest
   if (j < *count) pos[j] = rank[epid[j]];
 }
 
-// Column minima per instance (block = instance).
+// Column minima per instance as order keys, minkey[t * 16 + k] (columns
+// k < D; the entries must start at ~0).  Workgroup (t, y) of a T x Y grid
+// takes every Y-th stretch of kBlock rows of instance t, so that a large
+// instance is spread over Y workgroups.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_bcolmin(const float* __restrict__ erows,
                                                     const uint32_t* __restrict__ off,
@@ -475,7 +490,8 @@ __global__ __launch_bounds__(kBlock) void k_bcolmin(const float* __restrict__ er
   uint32_t best[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) best[k] = 0xFFFFFFFFu;
-  for (uint32_t i = off[t] + threadIdx.x; i < off[t + 1]; i += kBlock) {
+  const uint32_t step = kBlock * gridDim.y;
+  for (uint32_t i = off[t] + blockIdx.y * kBlock + threadIdx.x; i < off[t + 1]; i += step) {
 #pragma unroll
     for (int k = 0; k < D; ++k) best[k] = min(best[k], order_key(erows[(size_t)i * D + k]));
   }
@@ -486,7 +502,8 @@ __global__ __launch_bounds__(kBlock) void k_bcolmin(const float* __restrict__ er
     if ((threadIdx.x & 63) == 0) atomicMin(&red[k], v);
   }
   __syncthreads();
-  if (threadIdx.x < 16) minkey[(size_t)t * 16 + threadIdx.x] = threadIdx.x < D ? red[threadIdx.x] : 0u;
+  if (threadIdx.x < D && red[threadIdx.x] != 0xFFFFFFFFu)
+    atomicMin(&minkey[(size_t)t * 16 + threadIdx.x], red[threadIdx.x]);
 }
 
 // Per-instance task counts (blocks of P points) with a zero past the end,
@@ -512,11 +529,16 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // with workgroups of four waves it also holds the kernel to 64 VGPRs, so that
 // eight waves run per SIMD (with a grid of 2048 workgroups); the few values
 // that then live in scratch cost less than the doubled occupancy gains
-// (bmember_waves; the subgroup steps with eight points per workgroup from
-// d = 8 on need more registers than that).
+// (bmember_waves; not from d = 11 on, where the spills grow past what the
+// occupancy gains, nor for the subgroup steps with eight points per
+// workgroup from d = 8 on, which need more registers).  With eight waves per
+// workgroup (d = 13, 14) the kernel is held to 80 VGPRs instead, six waves
+// per SIMD, as many as its LDS allows: a large instance then runs as fast
+// as in the single-instance kernel (which needs 79).
 template <int D, int P, int NT, int ORD>
 constexpr int bmember_waves() {
-  return ORD != 0 && NT == 256 && !(ORD == kOrderSub && P == 8 && D >= 8) ? 8 : 1;
+  return ORD != 0 && NT == 256 && D <= 10 && !(ORD == kOrderSub && P == 8 && D >= 8) ? 8
+         : (ORD != 0 && NT == 512 && D <= 14 ? 6 : 1);
 }
 
 template <int D, int P, int NT, int ORD = 0>
@@ -547,7 +569,9 @@ void k_bmember(const float* __restrict__ erows,
   while (true) {
     if (threadIdx.x == 0) s_task = atomicAdd(work, 1u);
     __syncthreads();
-    const uint32_t task = s_task;
+    // the same for every lane: scalar, so that the task's instance, offsets
+    // and pointers below live in scalar registers
+    const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
     __syncthreads();
     if (task >= total) break;  // uniform
     const uint32_t t = seg_of(toff, T, task);

@@ -161,10 +161,13 @@ int exclusive_scan_dev(sky_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n
 }
 
 int seg_count_scan(sky_ctx* c, const uint8_t* flags, const uint32_t* seg, uint32_t T,
-                   uint32_t* cnt, uint32_t* off) {
-  hipLaunchKernelGGL(sky::k_seg_count, dim3(T + 1), dim3(sky::kPrimThreads), 0, c->stream, flags,
-                     seg, T, cnt);
-  LAUNCHCHK();
+                   uint32_t* cnt, uint32_t* off, uint32_t max_len) {
+  TRY(fill32(c, cnt, T + 1, 0));
+  if (T) {
+    hipLaunchKernelGGL(sky::k_seg_count, dim3(T, sky::seg_slices(max_len)),
+                       dim3(sky::kPrimThreads), 0, c->stream, flags, seg, cnt);
+    LAUNCHCHK();
+  }
   return exclusive_scan_dev(c, cnt, off, T + 1);
 }
 
@@ -533,6 +536,7 @@ int batch_setup(sky_ctx* c, const uint64_t* src, uint32_t n_inst, const uint32_t
       if ((int)d[t] == D && n[t] > 0) g.inst.push_back(t);
     g.T = (uint32_t)g.inst.size();
     g.off.assign(g.T + 1, 0);
+    g.max_n = 0;
     uint32_t* h_off = reinterpret_cast<uint32_t*>(stage(4ull * (g.T + 1)));
     uint64_t* h_src = reinterpret_cast<uint64_t*>(stage(8ull * g.T));
     uint2* h_tasks = reinterpret_cast<uint2*>(stage(8ull * ntask[D]));
@@ -545,6 +549,7 @@ int batch_setup(sky_ctx* c, const uint64_t* src, uint32_t n_inst, const uint32_t
       if (n[t] >= (1u << 29))
         return fail(c, SKY_ERR_CAPACITY, "instance %u: %u points (limit 2^29 - 1)", t, n[t]);
       g.off[k + 1] = g.off[k] + n[t];
+      g.max_n = std::max(g.max_n, n[t]);
       h_src[k] = src[t];
       h_md[k] = (max_d && max_d[t] && max_d[t] < (uint32_t)D) ? max_d[t] : (uint32_t)D;
       for (uint32_t s0 = 0; s0 < n[t]; s0 += kBlock) h_tasks[nt++] = make_uint2(k, g.off[k] + s0);

@@ -30,7 +30,7 @@ struct BatchGroup {
   int D = 0;
   std::vector<uint32_t> inst;  // caller's instance ids, in group order
   std::vector<uint32_t> off;   // point offsets of the instances (T + 1)
-  uint32_t T = 0, N = 0, ntasks = 0;
+  uint32_t T = 0, N = 0, ntasks = 0, max_n = 0;  // max_n: the largest instance
   DevBuf d_off, d_src, d_tasks, d_maxd;       // descriptors (d_src: row address per instance)
   DevBuf ext_pids, eoff, dom, cnt;            // results; cnt: counters
   mutable std::vector<uint32_t> eo_host;       // eoff read back by the last fetch
@@ -235,8 +235,10 @@ constexpr int kGroupCounters = 8;  // err, survivors, ext (id order), ext (score
 
 // Batch primitives without memset (core.hip): segmented counts of 0/1 flags
 // per instance and their exclusive scan; flagged select with a device count.
+// cnt[t] = flags set in segment t, off = their exclusive scan (T + 1
+// entries); max_len bounds the segments' lengths (it sets the parallelism).
 int seg_count_scan(sky_ctx* c, const uint8_t* flags, const uint32_t* seg, uint32_t T,
-                   uint32_t* cnt, uint32_t* off);
+                   uint32_t* cnt, uint32_t* off, uint32_t max_len);
 int select_dev(sky_ctx* c, const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t* count);
 int exclusive_scan_dev(sky_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n);
 
@@ -279,7 +281,7 @@ struct SdscChunk {
   const uint32_t* cand = nullptr;
   const uint32_t* off = nullptr;
   const uint2* tasks = nullptr;
-  uint32_t ntasks = 0, T = 0, N = 0;
+  uint32_t ntasks = 0, T = 0, N = 0, max_n = 0;  // max_n: the most candidates of a cuboid
   uint8_t* keep = nullptr;
   uint8_t* sky = nullptr;
   uint32_t* cnt = nullptr;  // kGroupCounters device words ([0] bit 1: NaN seen)

@@ -134,22 +134,29 @@ __global__ __launch_bounds__(kPrimThreads) void k_sel_scatter(const uint8_t* __r
   for (uint32_t m = f; m; m &= m - 1u) out[o++] = base + (uint32_t)(__ffs(m) - 1);
 }
 
-// Segmented count of set flags: cnt[t] = flags set in [off[t], off[t+1])
-// for t < T (one workgroup per instance), cnt[T] = 0.
+// Segmented count of set flags: cnt[t] += flags set in [off[t], off[t+1])
+// for t < T; cnt must start at zero.  Workgroup (t, y) of a grid of T x Y
+// takes every Y-th stretch of kPrimThreads flags of segment t, so that a
+// long segment is spread over Y workgroups.
 __global__ __launch_bounds__(kPrimThreads) void k_seg_count(const uint8_t* __restrict__ flags,
                                                             const uint32_t* __restrict__ off,
-                                                            uint32_t T, uint32_t* __restrict__ cnt) {
+                                                            uint32_t* __restrict__ cnt) {
   __shared__ uint32_t red[kPrimThreads / 64];
   const uint32_t t = blockIdx.x;
-  if (t == T) {
-    if (threadIdx.x == 0) cnt[T] = 0;
-    return;
-  }
+  const uint32_t step = kPrimThreads * gridDim.y;
   uint32_t s = 0;
-  for (uint32_t i = off[t] + threadIdx.x; i < off[t + 1]; i += kPrimThreads) s += flags[i] ? 1u : 0u;
+  for (uint32_t i = off[t] + blockIdx.y * kPrimThreads + threadIdx.x; i < off[t + 1]; i += step)
+    s += flags[i] ? 1u : 0u;
   uint32_t total;
   (void)block_excl_scan(s, red, &total);
-  if (threadIdx.x == 0) cnt[t] = total;
+  if (threadIdx.x == 0 && total) atomicAdd(cnt + t, total);
+}
+
+// Workgroups per segment for the segmented kernels: one per 16,384 items
+// of the longest segment, at most 64.
+inline unsigned seg_slices(uint32_t max_len) {
+  const uint32_t y = (max_len + 16383u) / 16384u;
+  return y < 1u ? 1u : (y > 64u ? 64u : y);
 }
 
 // Radix sort.  Tile b holds items b*kPrimTile + r*256 + t (r = round, t =

@@ -166,6 +166,7 @@ int sdsc_chunk(sky_ctx* c, const float* rows_dev, uint32_t d, uint32_t L, uint32
   k.ntasks = (uint32_t)tasks.size();
   k.T = T;
   k.N = N;
+  for (uint32_t q = 0; q < T; ++q) k.max_n = std::max(k.max_n, off[q + 1] - off[q]);
   k.keep = P<uint8_t>(c->sd_keep);
   k.sky = P<uint8_t>(c->sd_skyf);
   k.cnt = P<uint32_t>(c->sd_counters);
@@ -175,8 +176,10 @@ int sdsc_chunk(sky_ctx* c, const float* rows_dev, uint32_t d, uint32_t L, uint32
   // the kept candidates, in candidate order (ascending ids per cuboid)
   uint32_t* kept_dev = P<uint32_t>(c->sd_counters) + 8;
   TRY(select_dev(c, k.keep, N, P<uint32_t>(c->sd_sel), kept_dev));
-  TRY(seg_count_scan(c, k.keep, k.off, T, P<uint32_t>(c->sd_kcnt), P<uint32_t>(c->sd_koff)));
-  TRY(seg_count_scan(c, k.sky, k.off, T, P<uint32_t>(c->sd_scnt), P<uint32_t>(c->sd_soff)));
+  TRY(seg_count_scan(c, k.keep, k.off, T, P<uint32_t>(c->sd_kcnt), P<uint32_t>(c->sd_koff),
+                     k.max_n));
+  TRY(seg_count_scan(c, k.sky, k.off, T, P<uint32_t>(c->sd_scnt), P<uint32_t>(c->sd_soff),
+                     k.max_n));
   uint32_t kept = 0;
   TRY(read_u32(c, kept_dev, &kept));
   uint32_t err = 0;

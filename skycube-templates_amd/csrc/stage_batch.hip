@@ -109,7 +109,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
   LAUNCHCHK();
   TRY(select_dev(c, P<uint8_t>(B.alive), N, P<uint32_t>(B.s1idx), cnt + 1));
   uint32_t* s1off = P<uint32_t>(B.seg_b);
-  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off));
+  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off, g.max_n));
   hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
                      P<float>(B.srows), P<uint32_t>(B.skeys), P<uint32_t>(B.spid),
                      P<uint32_t>(B.s1idx), cnt + 1, P<float>(B.s1rows), P<uint32_t>(B.s1keys),
@@ -118,6 +118,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
   hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
                      P<float>(B.s1rows), s1off, T, P<uint32_t>(B.piv));
   LAUNCHCHK();
+  const dim3 gcol(T, sky::seg_slices(g.max_n));  // column minima, per instance
   if (batch_self_index(D, T)) {
     // survivors' codes, then the instance-major mask index and its table
     hipLaunchKernelGGL(sky::k_bcodes<D>, dim3(gN), dim3(kBlock), 0, c->stream,
@@ -132,7 +133,8 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
                        P<uint32_t>(B.isort), P<uint32_t>(B.s1keys), P<uint2>(B.codes), cnt + 1,
                        nullptr, T, P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr);
     LAUNCHCHK();
-    hipLaunchKernelGGL(sky::k_bcolmin<D>, dim3(T), dim3(kBlock), 0, c->stream,
+    TRY(fill32(c, P<uint32_t>(B.sminkey), (uint64_t)T * 16, 0xFFFFFFFFu));
+    hipLaunchKernelGGL(sky::k_bcolmin<D>, gcol, dim3(kBlock), 0, c->stream,
                        P<float>(B.s1rows), s1off, P<uint32_t>(B.sminkey));
     LAUNCHCHK();
     hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
@@ -157,7 +159,7 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
   // extended skyline in id order (the result) and in score order
   TRY(select_dev(c, P<uint8_t>(B.flag_pt), N, P<uint32_t>(g.ext_pids), cnt + 2));
   uint32_t* eoff = P<uint32_t>(g.eoff);
-  TRY(seg_count_scan(c, P<uint8_t>(B.flag_pt), off, T, P<uint32_t>(B.seg_a), eoff));
+  TRY(seg_count_scan(c, P<uint8_t>(B.flag_pt), off, T, P<uint32_t>(B.seg_a), eoff, g.max_n));
   TRY(select_dev(c, P<uint8_t>(B.flag_s1), N, P<uint32_t>(B.eidx), cnt + 3));
   hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
                      P<float>(B.s1rows), nullptr, P<uint32_t>(B.s1pid), P<uint32_t>(B.eidx),
@@ -169,7 +171,8 @@ int batch_group_d(sky_ctx* c, BatchGroup& g) {
   hipLaunchKernelGGL(sky::k_bpos, dim3(gN), dim3(kBlock), 0, c->stream, P<uint32_t>(B.epid),
                      cnt + 3, P<uint32_t>(B.rank), P<uint32_t>(B.pos));
   LAUNCHCHK();
-  hipLaunchKernelGGL(sky::k_bcolmin<D>, dim3(T), dim3(kBlock), 0, c->stream, P<float>(B.erows),
+  TRY(fill32(c, P<uint32_t>(B.minkey), (uint64_t)T * 16, 0xFFFFFFFFu));
+  hipLaunchKernelGGL(sky::k_bcolmin<D>, gcol, dim3(kBlock), 0, c->stream, P<float>(B.erows),
                      eoff, P<uint32_t>(B.minkey));
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_bpivots<D>, dim3(T * D), dim3(kBlock), 0, c->stream,
@@ -256,7 +259,7 @@ int sdsc_chunk_d(sky_ctx* c, const SdscChunk& k) {
   LAUNCHCHK();
   TRY(select_dev(c, P<uint8_t>(B.alive), N, P<uint32_t>(B.s1idx), cnt + 1));
   uint32_t* s1off = P<uint32_t>(B.seg_b);
-  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off));
+  TRY(seg_count_scan(c, P<uint8_t>(B.alive), off, T, P<uint32_t>(B.seg_a), s1off, k.max_n));
   hipLaunchKernelGGL(sky::k_bgather_idx<D>, dim3(gN), dim3(kBlock), 0, c->stream,
                      P<float>(B.srows), P<uint32_t>(B.skeys), P<uint32_t>(B.spid),
                      P<uint32_t>(B.s1idx), cnt + 1, P<float>(B.s1rows), P<uint32_t>(B.s1keys),
