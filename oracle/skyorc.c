@@ -755,37 +755,47 @@ uint64_t orc_score_tie_count(const float *rows, int d, int max_d,
     }
   /* |fl(sum) - sum| <= (k-1) u sum|x| for k terms, u = 2^-24; two sums. */
   const double thr = 2.02 * (d - 1) * (double)d * (double)vmax * ldexp(1.0, -24);
+  /* points a in blocks of ORC_BLOCK, the candidates b outer: each
+   * candidate row is read once per block (from a contiguous copy) */
+  float *cand = gather_rows(rows, d, ext, n_ext);
+  if (!cand)
+    return UINT64_MAX;
   uint64_t total = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
-  for (uint32_t a = 0; a < n_ext; ++a) {
-    const float *p = rows + (size_t)ext[a] * d;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+  for (uint32_t a0 = 0; a0 < n_ext; a0 += ORC_BLOCK) {
+    const uint32_t na = n_ext - a0 < ORC_BLOCK ? n_ext - a0 : ORC_BLOCK;
     for (uint32_t b = 0; b < n_ext; ++b) {
-      if (a == b)
-        continue;
-      const float *q = rows + (size_t)ext[b] * d;
-      uint32_t lt, eq;
-      pair_masks(q, p, d, &lt, &eq);
-      if (lt == 0)
-        continue;
-      double gap = INFINITY;
-      for (uint32_t t = lt; t; t &= t - 1) {
-        const int i = __builtin_ctz(t);
-        const double g = (double)p[i] - (double)q[i];
-        if (g < gap)
-          gap = g;
-      }
-      if (gap > thr)
-        continue;
-      const uint32_t le = lt | eq;
-      /* enumerate the subsets s of le that meet lt */
-      for (uint32_t s = le; s; s = (s - 1) & le) {
-        if (!(s & lt) || __builtin_popcount(s) > max_d)
+      const float *q = cand + (size_t)b * d;
+      for (uint32_t k = 0; k < na; ++k) {
+        const uint32_t a = a0 + k;
+        if (a == b)
           continue;
-        if (score_of(q, s, d) == score_of(p, s, d))
-          total++;
+        const float *p = cand + (size_t)a * d;
+        uint32_t lt, eq;
+        pair_masks(q, p, d, &lt, &eq);
+        if (lt == 0)
+          continue;
+        double gap = INFINITY;
+        for (uint32_t t = lt; t; t &= t - 1) {
+          const int i = __builtin_ctz(t);
+          const double g = (double)p[i] - (double)q[i];
+          if (g < gap)
+            gap = g;
+        }
+        if (gap > thr)
+          continue;
+        const uint32_t le = lt | eq;
+        /* enumerate the subsets s of le that meet lt */
+        for (uint32_t s = le; s; s = (s - 1) & le) {
+          if (!(s & lt) || __builtin_popcount(s) > max_d)
+            continue;
+          if (score_of(q, s, d) == score_of(p, s, d))
+            total++;
+        }
       }
     }
   }
+  free(cand);
   return total;
 }
 

@@ -153,7 +153,10 @@ def score_tie_count(x, ext, max_d: int | None = None) -> int:
     n, d = x.shape
     max_d = d if max_d is None else max_d
     ext = np.ascontiguousarray(ext, dtype=np.uint32)
-    return int(lib().orc_score_tie_count(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p), ext.size))
+    count = int(lib().orc_score_tie_count(_ptr(x, _f32p), d, max_d, _ptr(ext, _u32p), ext.size))
+    if count == 2**64 - 1:
+        raise MemoryError("score_tie_count: no memory for the candidate rows")
+    return count
 
 
 def score_tie_unresolved(x, ext, max_d: int | None = None) -> int:
