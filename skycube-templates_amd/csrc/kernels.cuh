@@ -978,7 +978,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     constexpr int NW = NT / 64;
     constexpr int WPP = NW;
     const int wv = tid >> 6;
-    const uint32_t mine = s_ok;
+    const uint32_t mine = __builtin_amdgcn_readfirstlane(s_ok);
     const uint32_t ng = (m + 63) / 64;
     for (uint32_t g0 = (uint32_t)(wv % WPP) * 64; g0 < ng; g0 += (uint32_t)WPP * 64) {
       const uint32_t g = g0 + (uint32_t)lane;
@@ -1006,7 +1006,9 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 #pragma unroll
         for (int r = 0; r < kLive; ++r) {
           const int L = Ls[r] < 0 ? 0 : Ls[r];
-          gal[r] = Ls[r] < 0 ? 0u : (uint32_t)__shfl((int)al, L, 64);
+          // lane L is the same for every lane: a scalar read, so that the
+          // point tests below branch on a scalar mask
+          gal[r] = Ls[r] < 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)al, L);
           jj[r] = (g0 + (uint32_t)L) * 64u + (uint32_t)lane;
           qc[r] = ix.gcode[min(jj[r], m - 1u)];
         }
@@ -1058,7 +1060,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     // round (or, at the end, whatever is left).
     constexpr int NW = NT / 64;
     const int wv = tid >> 6;
-    const uint32_t mine = s_ok;
+    const uint32_t mine = __builtin_amdgcn_readfirstlane(s_ok);  // uniform: scalar branches
     const uint32_t ng = (m + 63) / 64;
     uint32_t* l1 = wlist[wv];            // < 72 entries
     uint32_t* l2 = wlist[wv] + kList1;   // < 64 + 64 P entries: subgroup << 3 | point
@@ -1245,7 +1247,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       A0[pp][k] = low_closure(a0, D);
       A1[pp][k] = low_closure(A1[pp][k] | a0, D);
       uint32_t t = TE[pp][k];
-      const uint32_t z = s_zm[pp];
+      const uint32_t z = __builtin_amdgcn_readfirstlane(s_zm[pp]);
       if (D > 0) t |= (z & 1u) ? (t & 0x55555555u) << 1 : (t & 0xAAAAAAAAu) >> 1;
       if (D > 1) t |= (z & 2u) ? (t & 0x33333333u) << 2 : (t & 0xCCCCCCCCu) >> 2;
       if (D > 2) t |= (z & 4u) ? (t & 0x0F0F0F0Fu) << 4 : (t & 0xF0F0F0F0u) >> 4;
@@ -1267,7 +1269,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         for (int pp = 0; pp < P; ++pp) {
           A0[pp][lo] |= A0[pp][hi];
           A1[pp][lo] |= A1[pp][hi];
-          if (s_zm[pp] >> (lv + 5) & 1u)
+          if (__builtin_amdgcn_readfirstlane(s_zm[pp]) >> (lv + 5) & 1u)
             TE[pp][hi] |= TE[pp][lo];
           else
             TE[pp][lo] |= TE[pp][hi];
@@ -1279,7 +1281,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         const int lo = ((i >> lv) << (lv + 1)) | (i & ((1 << lv) - 1));
 #pragma unroll
         for (int pp = 0; pp < P; ++pp)
-          if (lv < __popc(s_zm[pp])) U[pp][lo] |= U[pp][lo | (1 << lv)];
+          if (lv < __popc(__builtin_amdgcn_readfirstlane(s_zm[pp])))
+            U[pp][lo] |= U[pp][lo | (1 << lv)];
       }
     }
   }
@@ -1288,8 +1291,9 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   // per point: final words; X bits are parked in TE for the second pass
 #pragma unroll 1
   for (int pp = 0; pp < P; ++pp) {
-    const uint32_t z = s_zm[pp];
-    const uint32_t rg = s_rgt[pp];
+    // the point's values are the same for every lane: scalar registers
+    const uint32_t z = __builtin_amdgcn_readfirstlane(s_zm[pp]);
+    const uint32_t rg = __builtin_amdgcn_readfirstlane(s_rgt[pp]);
     const uint32_t lowR = ~z & 31u;
     uint32_t m0 = 0, m1 = 0;  // bits b whose low columns outside Z number 0 / 1
     low_subsets(~lowR & 31u, &m0, &m1);
