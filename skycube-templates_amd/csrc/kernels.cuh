@@ -154,7 +154,6 @@ __device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
                                              const uint32_t* __restrict__ keys,
                                              const uint32_t* __restrict__ masks, uint32_t lo0,
                                              uint32_t hi0, uint32_t i, int need_dom, uint32_t* q) {
-  constexpr uint32_t FULL = (1u << D) - 1u;
   const int lane = threadIdx.x & 63;
   float p[D];
 #pragma unroll
@@ -190,15 +189,17 @@ __device__ __forceinline__ uint8_t self_scan(const float* __restrict__ rows,
       bool strict = false, dom = false;
       for (int e = lane; e < qn; e += 64) {
         const float* qr = rows + (size_t)q[e] * D;
-        uint32_t lt = 0, eq = 0;
+        // flags, not masks: the compares' lane masks combine as scalars
+        bool all_lt = true, any_lt = false, all_le = true;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           const float v = qr[k];
-          lt |= (uint32_t)(v < p[k]) << k;
-          eq |= (uint32_t)(v == p[k]) << k;
+          all_lt &= v < p[k];
+          any_lt |= v < p[k];
+          all_le &= v <= p[k];
         }
-        strict |= lt == FULL;
-        dom |= lt != 0 && (lt | eq) == FULL;
+        strict |= all_lt;
+        dom |= any_lt && all_le;
       }
       qn = 0;
       if (__ballot(strict)) {
@@ -375,15 +376,16 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
           const uint32_t qi = bidx[j];
           if (qi != i) {
             const float* qr = rows + (size_t)qi * D;
-            uint32_t lt = 0, eq = 0;
+            bool all_lt = true, any_lt = false, all_le = true;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
               const float v = qr[k];
-              lt |= (uint32_t)(v < p[k]) << k;
-              eq |= (uint32_t)(v == p[k]) << k;
+              all_lt &= v < p[k];
+              any_lt |= v < p[k];
+              all_le &= v <= p[k];
             }
-            strict = lt == FULL;
-            dom = lt != 0 && (lt | eq) == FULL;
+            strict = all_lt;
+            dom = any_lt && all_le;
           }
         }
       }
@@ -397,8 +399,7 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
   return st;
 }
 
-// Held to 64 VGPRs (eight waves per SIMD): at d = 15, 16 a few values then
-// live in scratch, which costs less than the lost occupancy did.
+// Held to 64 VGPRs (eight waves per SIMD).
 template <int D>
 __global__ __attribute__((amdgpu_waves_per_eu(8))) __launch_bounds__(kBlock) void k_self_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
@@ -662,12 +663,14 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
   float qv[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) qv[k] = q[k];
+  // shifted in from the top column down: no per-column bit constants,
+  // which the compiler would otherwise keep in registers
   lt = 0;
   eq = 0;
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    lt |= (uint32_t)(qv[k] < p[k]) << k;
-    eq |= (uint32_t)(qv[k] == p[k]) << k;
+  for (int k = D - 1; k >= 0; --k) {
+    lt = (lt << 1) | (uint32_t)(qv[k] < p[k]);
+    eq = (eq << 1) | (uint32_t)(qv[k] == p[k]);
   }
 }
 
