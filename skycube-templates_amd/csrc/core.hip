@@ -208,6 +208,7 @@ int skyi::run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, ui
   const uint32_t W = sky_words_per_point(d);
   if (max_d == 0 || max_d > d) max_d = d;
   *n_ext = 0;
+  c->res_rowcnt = false;
   if (n == 0) {
     c->has_mdmc = true;
     c->res_n_ext = 0;
@@ -271,6 +272,7 @@ int skyi::run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, ui
   c->res_n_ext = ne;
   c->res_d = d;
   c->res_W = W;
+  c->res_rowcnt = ne && !c->carve;
   *n_ext = ne;
   return SKY_OK;
 }
@@ -372,8 +374,15 @@ struct ToU64 {
 // emitted in point-major order and sorted stably by key, so the ids of a
 // group ascend; runs of equal keys are the groups.  The extra always-zero
 // word of d >= 5 is one more group (W, 0) holding every point, added when
-// the groups are copied out.
-int build_hashcube(sky_ctx* c) {
+// the groups are copied out.  The entries of each tile of 4096 words are
+// counted by a pass over the bitmaps, or, where the membership kernel counted
+// each row's as it stored it (res_rowcnt), taken from those counts.
+//
+// If pids is given with room for every entry, the ids are copied into it
+// on the context's copy stream as soon as they are sorted, while the groups
+// are still being formed (*early: the copy was started; the caller waits
+// for the copy stream).
+int build_hashcube_impl(sky_ctx* c, uint32_t* pids, uint64_t cap_entries, bool* early) {
   const uint32_t ne = c->res_n_ext, W = c->res_W;
   c->hc_groups = c->hc_entries = 0;
   if (ne == 0) {
@@ -383,34 +392,62 @@ int build_hashcube(sky_ctx* c) {
   const uint64_t words = (uint64_t)ne * W;
   const uint64_t tiles = (words + sky::kHcTile - 1) / sky::kHcTile;
   if (tiles >= 0xFFFFFFFFull) return fail(c, SKY_ERR_CAPACITY, "HashCube export too large");
-  TRY(ensure(c, c->hc_cnt, sizeof(uint32_t) * (tiles + 1)));
-  TRY(ensure(c, c->hc_runs, prim_tmp_bytes(tiles + 1)));
+  int wbits = 0;
+  while ((1u << wbits) < W) ++wbits;
   TRY(ensure(c, c->counter, 64 * sizeof(uint32_t)));
   uint32_t* total_dev = P<uint32_t>(c->counter) + 48;
-  hipLaunchKernelGGL(sky::k_hc_count, dim3((unsigned)tiles), dim3(256), 0, c->stream,
-                     P<uint32_t>(c->dom), words, P<uint32_t>(c->hc_cnt));
-  LAUNCHCHK();
-  HIPCHK(prim_scan(P<uint32_t>(c->hc_cnt), P<uint32_t>(c->hc_cnt), (uint32_t)tiles, total_dev,
-                   c->hc_runs.p, c->stream));
+  const uint32_t* toff = nullptr;
+  int tshift = 0;
+  if (c->res_rowcnt) {
+    // exclusive scan of the row counts: a tile's offset is its first row's
+    TRY(ensure(c, c->hc_runs, prim_tmp_bytes(ne + 1)));
+    HIPCHK(prim_scan(P<uint32_t>(c->hc_off), P<uint32_t>(c->hc_off), ne, total_dev,
+                     c->hc_runs.p, c->stream));
+    toff = P<uint32_t>(c->hc_off);
+    tshift = 12 - wbits;
+    c->res_rowcnt = false;  // hc_off holds offsets now
+  } else {
+    TRY(ensure(c, c->hc_cnt, sizeof(uint32_t) * (tiles + 1)));
+    TRY(ensure(c, c->hc_runs, prim_tmp_bytes(tiles + 1)));
+    hipLaunchKernelGGL(sky::k_hc_count, dim3((unsigned)tiles), dim3(256), 0, c->stream,
+                       P<uint32_t>(c->dom), words, P<uint32_t>(c->hc_cnt));
+    LAUNCHCHK();
+    HIPCHK(prim_scan(P<uint32_t>(c->hc_cnt), P<uint32_t>(c->hc_cnt), (uint32_t)tiles, total_dev,
+                     c->hc_runs.p, c->stream));
+    toff = P<uint32_t>(c->hc_cnt);
+  }
   uint32_t total = 0;
   HIPCHK(hipMemcpyAsync(&total, total_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if ((uint64_t)total + (c->res_d >= 5 ? ne : 0) > 0xFFFFFFFFull)
+  const bool extra = c->res_d >= 5;
+  if ((uint64_t)total + (extra ? ne : 0) > 0xFFFFFFFFull)
     return fail(c, SKY_ERR_CAPACITY, "HashCube export of more than 2^32 entries");
   TRY(ensure(c, c->hc_key_a, sizeof(uint64_t) * std::max<uint32_t>(total, 1)));
   TRY(ensure(c, c->hc_key_b, sizeof(uint64_t) * std::max<uint32_t>(total, 1)));
   TRY(ensure(c, c->hc_val_a, sizeof(uint32_t) * std::max<uint32_t>(total, 1)));
   TRY(ensure(c, c->hc_val_b, sizeof(uint32_t) * std::max<uint32_t>(total, 1)));
-  int wbits = 0;
-  while ((1u << wbits) < W) ++wbits;
   hipLaunchKernelGGL(sky::k_hc_emit, dim3((unsigned)tiles), dim3(256), 0, c->stream,
-                     P<uint32_t>(c->dom), words, wbits, P<uint32_t>(c->ext_pids),
-                     P<uint32_t>(c->hc_cnt), P<uint64_t>(c->hc_key_a), P<uint32_t>(c->hc_val_a));
+                     P<uint32_t>(c->dom), words, wbits, P<uint32_t>(c->ext_pids), toff, tshift,
+                     P<uint64_t>(c->hc_key_a), P<uint32_t>(c->hc_val_a));
   LAUNCHCHK();
   uint32_t ng = 0;
   if (total) {
     TRY(sort_pairs(c, P<uint64_t>(c->hc_key_a), P<uint64_t>(c->hc_key_b),
                    P<uint32_t>(c->hc_val_a), P<uint32_t>(c->hc_val_b), total, 32 + wbits));
+    if (pids && cap_entries >= (uint64_t)total + (extra ? ne : 0)) {
+      if (!c->copy_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->sorted_ev, hipEventDisableTiming));
+      }
+      HIPCHK(hipEventRecord(c->sorted_ev, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->copy_stream, c->sorted_ev, 0));
+      *early = true;
+      HIPCHK(hipMemcpyAsync(pids, c->hc_val_b.p, sizeof(uint32_t) * total, hipMemcpyDeviceToHost,
+                            c->copy_stream));
+      if (extra)  // the extra word's group: every point, ascending
+        HIPCHK(hipMemcpyAsync(pids + total, c->ext_pids.p, sizeof(uint32_t) * ne,
+                              hipMemcpyDeviceToHost, c->copy_stream));
+    }
     // runs of equal keys = groups
     TRY(ensure(c, c->hc_ukey, sizeof(uint64_t) * total));
     uint32_t* run_len = P<uint32_t>(c->hc_val_a);  // reuse: the unsorted values are done
@@ -439,12 +476,19 @@ int build_hashcube(sky_ctx* c) {
     LAUNCHCHK();
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  const bool extra = c->res_d >= 5;
   c->hc_groups = ng + (extra ? 1 : 0);
   c->hc_entries = (uint64_t)total + (extra ? ne : 0);
   c->hc_sorted = total;
   c->has_hc = true;
   return SKY_OK;
+}
+
+// build_hashcube_impl; an early copy it started has ended if it fails.
+int build_hashcube(sky_ctx* c, uint32_t* pids, uint64_t cap_entries, bool* early) {
+  *early = false;
+  const int rc = build_hashcube_impl(c, pids, cap_entries, early);
+  if (rc != SKY_OK && *early) (void)hipStreamSynchronize(c->copy_stream);
+  return rc;
 }
 
 // Forgets the current batch (graph included); the buffers stay for the next.
@@ -736,6 +780,11 @@ void sky_ctx_destroy(sky_ctx* c) {
   free_all(c);
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->copy_stream) {
+    (void)hipStreamSynchronize(c->copy_stream);
+    (void)hipStreamDestroy(c->copy_stream);
+  }
+  if (c->sorted_ev) (void)hipEventDestroy(c->sorted_ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -962,9 +1011,13 @@ int sky_mdmc_hashcube(sky_ctx* c, uint64_t* n_groups, uint64_t* n_entries, uint3
   TRY(check_ctx(c));
   if (!n_groups || !n_entries) return fail(c, SKY_ERR_ARG, "null argument");
   if (!c->has_mdmc) return fail(c, SKY_ERR_STATE, "no MDMC result is held");
-  if (!c->has_hc) TRY(build_hashcube(c));
+  bool early = false;  // the ids are already on their way into pids
+  if (!c->has_hc) TRY(build_hashcube(c, group_word && group_value && group_off ? pids : nullptr,
+                                     cap_entries, &early));
   *n_groups = c->hc_groups;
   *n_entries = c->hc_entries;
+  if (early && cap_groups < c->hc_groups)
+    HIPCHK(hipStreamSynchronize(c->copy_stream));
   if (!group_word || !group_value || !group_off || !pids) return SKY_OK;
   if (cap_groups < c->hc_groups || cap_entries < c->hc_entries)
     return fail(c, SKY_ERR_CAPACITY, "HashCube needs %llu groups and %llu entries",
@@ -978,13 +1031,16 @@ int sky_mdmc_hashcube(sky_ctx* c, uint64_t* n_groups, uint64_t* n_entries, uint3
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(group_off, c->hc_goff.p, sizeof(uint64_t) * nsg,
                           hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(pids, c->hc_val_b.p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost,
-                          c->stream));
+    if (!early)
+      HIPCHK(hipMemcpyAsync(pids, c->hc_val_b.p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost,
+                            c->stream));
   }
-  if (ng > nsg)  // the extra word's group: every point, ascending
+  if (ng > nsg && !early)  // the extra word's group: every point, ascending
     HIPCHK(hipMemcpyAsync(pids + ns, c->ext_pids.p, sizeof(uint32_t) * c->res_n_ext,
                           hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  const hipError_t e1 = hipStreamSynchronize(c->stream);
+  if (early) HIPCHK(hipStreamSynchronize(c->copy_stream));
+  HIPCHK(e1);
   if (ng > nsg) {
     group_word[nsg] = c->res_W;
     group_value[nsg] = 0;

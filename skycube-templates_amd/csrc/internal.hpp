@@ -106,6 +106,9 @@ struct sky_ctx {
   // grouping of the last HashCube export
   bool has_hc = false;
   uint64_t hc_groups = 0, hc_entries = 0, hc_sorted = 0;
+  // the export's early copy of the ids (created on first use)
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t sorted_ev = nullptr;
   // stage timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
@@ -113,6 +116,7 @@ struct sky_ctx {
   // last MDMC result
   bool has_mdmc = false;
   uint32_t res_n_ext = 0, res_d = 0, res_W = 0;
+  bool res_rowcnt = false;  // hc_off holds the result's per-row entry counts
   // set during sky_mdmc_device_noalloc: buffers come from the scratch array
   skyi::Carve* carve = nullptr;
   // the last SDSC lattice (sdsc.hip): per cuboid s an ascending id list (sky

@@ -714,6 +714,9 @@ struct CandIndex {
   // if set, a point with more minimum columns than the kernel's boundary
   // table holds is not computed but appended to defer[1 ..] (count defer[0])
   uint32_t* defer = nullptr;
+  // if set, rowcnt[row] gains the number of the row's words that are not
+  // all ones (the HashCube export's entries of that point)
+  uint32_t* rowcnt = nullptr;
 };
 
 // Down-closure of the P bitmaps A (W words each) in place; caller syncs
@@ -1420,8 +1423,17 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     // streaming stores: the bitmaps are not read again here, and keeping
     // them out of L2 leaves it to the candidate codes the other workgroups
     // are scanning
-    for (int k = tid; k < W; k += NT)
-      __builtin_nontemporal_store(A1[pp][k], &dom[(size_t)row * W + k]);
+    uint32_t cnt = 0;
+    for (int k = tid; k < W; k += NT) {
+      const uint32_t v = A1[pp][k];
+      cnt += v != 0xFFFFFFFFu;
+      __builtin_nontemporal_store(v, &dom[(size_t)row * W + k]);
+    }
+    if (ix.rowcnt) {  // uniform
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+      if ((tid & 63) == 0 && cnt) atomicAdd(&ix.rowcnt[row], cnt);
+    }
   }
   __syncthreads();  // the LDS may be reused by the caller's next block
 }
@@ -1539,7 +1551,8 @@ __global__ void k_group_sum(const uint2* __restrict__ codes, const uint32_t* __r
 // of contiguous memory.  Entries are the words that are not all ones.
 
 constexpr int kHcItems = 16;
-constexpr uint32_t kHcTile = 256 * kHcItems;
+constexpr uint32_t kHcTile = 256 * kHcItems;  // 2^12 words
+static_assert(kHcTile == 4096, "per-row offsets assume tiles of 2^12 words");
 
 __device__ __forceinline__ void hc_load(const uint32_t* __restrict__ dom, uint64_t total,
                                         uint64_t k0, uint32_t (&v)[kHcItems]) {
@@ -1578,10 +1591,12 @@ __global__ __launch_bounds__(256) void k_hc_count(const uint32_t* __restrict__ d
 }
 
 // Entries (key = j << 32 | value, id of point i) of word k = i * W + j, in
-// ascending k from the tile's scanned offset (W = 2^wbits).
+// ascending k from the tile's scanned offset toff[tile << tshift] (W =
+// 2^wbits; tshift 0 with per-tile offsets, 12 - wbits with per-row ones: a
+// tile holds 2^(12 - wbits) whole rows).
 __global__ __launch_bounds__(256) void k_hc_emit(const uint32_t* __restrict__ dom, uint64_t total,
                                                  int wbits, const uint32_t* __restrict__ ids,
-                                                 const uint32_t* __restrict__ toff,
+                                                 const uint32_t* __restrict__ toff, int tshift,
                                                  uint64_t* __restrict__ key,
                                                  uint32_t* __restrict__ val) {
   __shared__ uint32_t red[4];
@@ -1601,7 +1616,7 @@ __global__ __launch_bounds__(256) void k_hc_emit(const uint32_t* __restrict__ do
   }
   if (lane == 63) red[wv] = inc;
   __syncthreads();
-  uint32_t o = toff[blockIdx.x] + inc - s;
+  uint32_t o = toff[(uint64_t)blockIdx.x << tshift] + inc - s;
   for (int w = 0; w < wv; ++w) o += red[w];
   const uint64_t wmask = (1ull << wbits) - 1ull;
 #pragma unroll

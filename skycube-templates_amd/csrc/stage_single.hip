@@ -255,6 +255,13 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
     ix.gcode = P<uint2>(c->bcode);
     ix.gidx = P<uint32_t>(c->bidx);
   }
+  if (!c->carve) {  // the allocation-free call builds no HashCube
+    // per-row entry counts of the HashCube export, counted as the rows are
+    // stored
+    TRY(ensure(c, c->hc_off, sizeof(uint32_t) * ((size_t)ne + 1)));
+    TRY(fill32(c, c->hc_off.p, ne, 0u));
+    ix.rowcnt = P<uint32_t>(c->hc_off);
+  }
   TRY(mark(c, SKY_STAGE_MEMBER));  // the membership kernel launches alone
   if (order == sky::kOrderSub)
     return launch_member<D, PP, NT, sky::kOrderSub>(c, ne, max_d, minkey, cover, ix);

@@ -185,6 +185,32 @@ def test_hashcube_export(ctx, d, max_d, n):
                               np.sort(skycube_amd.members(dom, ext, s)))
 
 
+@pytest.mark.parametrize("d,max_d,n", [(4, 4, 700), (10, 6, 2500), (16, 8, 1500)])
+def test_hashcube_export_single_call(ctx, d, max_d, n):
+    """The export into caller buffers sized in advance (one call: the ids
+    start copying as soon as they are sorted, and the per-row entry counts
+    come from the membership kernel) equals the host grouping."""
+    x = datagen.integer_ties(n, d, 7100 + d)
+    _say(f"hashcube single call d={d} max_d={max_d} n={n}")
+    ext, dom = ctx.mdmc(x, max_d)
+    W = dom.shape[1]
+    cap_e = len(ext) * (W + 1) + 1
+    gw = np.zeros(cap_e + 1, dtype=np.uint32)
+    gv = np.zeros(cap_e + 1, dtype=np.uint32)
+    go = np.zeros(cap_e + 2, dtype=np.uint64)
+    ids = np.zeros(cap_e, dtype=np.uint32)
+    g, t = ctx.hashcube_into(gw.ctypes.data, gv.ctypes.data, go.ctypes.data, ids.ctypes.data,
+                             cap_e + 1, cap_e)
+    hw, hv, ho, hids = skycube_amd.hashcube_from_bitmaps(ext, dom, d)
+    assert np.array_equal(gw[:g], hw) and np.array_equal(gv[:g], hv)
+    assert np.array_equal(go[:g + 1], ho) and np.array_equal(ids[:t], hids)
+    # the copies again, from the groups already built
+    ids2 = np.zeros(cap_e, dtype=np.uint32)
+    assert ctx.hashcube_into(gw.ctypes.data, gv.ctypes.data, go.ctypes.data, ids2.ctypes.data,
+                             cap_e + 1, cap_e) == (g, t)
+    assert np.array_equal(ids2[:t], hids)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("d", [3, 6, 9, 12, 14, 16])
 def test_mdmc_member_candidate_order(d, mode):
