@@ -226,8 +226,16 @@ int skyi::run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, ui
                      P<uint8_t>(c->status), P<uint32_t>(c->s1pid), m, P<uint8_t>(c->flag));
   LAUNCHCHK();
   TRY(select_flagged(c, P<uint8_t>(c->flag), n, P<uint32_t>(c->ext_pids)));
-  uint32_t ne = 0;
-  TRY(copy_count(c, &ne));
+  // the extended skyline's size, and the tied sample values the split's
+  // pivot pass counted (counter[2])
+  uint32_t cnt3[3] = {0u, 0u, 0u};
+  HIPCHK(hipMemcpyAsync(cnt3, P<uint32_t>(c->counter), sizeof(cnt3), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const uint32_t ne = cnt3[0];
+  // more than 1 in 64 sample values tied: the membership keeps its tie
+  // lattices in every workgroup
+  const bool ties = (uint64_t)cnt3[2] * 64 > (uint64_t)std::min<uint32_t>(m, 2048) * d;
   if (ne >= (1u << 29))
     return fail(c, SKY_ERR_CAPACITY, "extended skyline of %u points (limit 2^29 - 1)", ne);
   TRY(ensure(c, c->erows, sizeof(float) * (size_t)ne * d));
@@ -260,7 +268,7 @@ int skyi::run_mdmc(sky_ctx* c, const float* rows_dev, uint32_t n, uint32_t d, ui
     hipLaunchKernelGGL(sky::k_pos, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
                        P<uint32_t>(c->epid), ne, P<uint32_t>(c->key_a), P<uint32_t>(c->pos));
     LAUNCHCHK();
-    SKY_DISPATCH_D(d, rc = member_d<D>(c, ne, max_d, minkey));
+    SKY_DISPATCH_D(d, rc = member_d<D>(c, ne, max_d, minkey, ties));
     TRY(rc);
   } else {
     TRY(mark(c, SKY_STAGE_MEMBER));

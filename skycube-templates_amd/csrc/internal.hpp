@@ -268,7 +268,7 @@ int split_d(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, sky:
 // Membership of the ne extended-skyline points (rows in c->erows, ids in
 // c->epid, output rows in c->pos): grid codes, then the membership launch.
 template <int D>
-int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey);
+int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey, bool ties);
 
 // One batch group's whole pass, enqueued on the context stream.
 template <int D>

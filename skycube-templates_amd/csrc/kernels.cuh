@@ -588,9 +588,14 @@ constexpr int member_threads() {
 // codes; good ones make the codes selective.
 constexpr int kPivotSplit = 8;  // blocks per column of k_pivots
 
+//
+// If dups is given it gains the number of sample values equal to an
+// earlier one of their column and above the sample's minimum (how tied the
+// columns are away from their minima, where ties are boundary columns).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ erows, uint32_t m,
-                                                   uint32_t* __restrict__ piv) {
+                                                   uint32_t* __restrict__ piv,
+                                                   uint32_t* __restrict__ dups) {
   constexpr int S = 2048;
   __shared__ uint32_t val[S];
   const int col = blockIdx.x / kPivotSplit;
@@ -601,13 +606,23 @@ __global__ __launch_bounds__(kBlock) void k_pivots(const float* __restrict__ ero
   __syncthreads();
   // each sample value's rank (ties broken by position, so ranks are
   // distinct); the values of ranks (b+1)*cnt/16 are the pivots
+  uint32_t nd = 0;
   for (uint32_t i = part * kBlock + threadIdx.x; i < cnt; i += kBlock * kPivotSplit) {
     const uint32_t v = val[i];
-    uint32_t r = 0;
-    for (uint32_t j = 0; j < cnt; ++j) r += (val[j] < v) || (val[j] == v && j < i);
+    uint32_t r = 0, e = 0;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      r += (val[j] < v) || (val[j] == v && j < i);
+      e += val[j] == v && j < i;
+    }
+    nd += e != 0 && r != e;  // r - e values are below v
 #pragma unroll 1
     for (int b = 0; b < kCodePivots; ++b)
       if ((uint32_t)((b + 1) * cnt / (kCodePivots + 1)) == r) piv[col * kCodePivots + b] = v;
+  }
+  if (dups) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nd += __shfl_xor(nd, o, 64);
+    if ((threadIdx.x & 63) == 0 && nd) atomicAdd(dups, nd);
   }
 }
 
@@ -637,18 +652,23 @@ __global__ void k_codes(const float* __restrict__ erows, uint32_t m,
 }
 
 // Marks one (q, p) pair with masks lt, eq into the structures of point pp.
-template <int W, int NU>
-__device__ __forceinline__ void mark_pair(uint32_t (*A0)[W], uint32_t (*A1)[W], uint32_t (*TE)[W],
-                                          uint32_t (*U)[NU], int pp, uint32_t lt, uint32_t eq,
-                                          uint32_t z) {
+// Without the tie lattices (TIES false), a pair with a residual tie only
+// flags its point (tie[pp]), which is then computed again with them.
+template <bool TIES, int W, int NU, int WT>
+__device__ __forceinline__ void mark_pair(uint32_t (*A0)[W], uint32_t (*A1)[WT],
+                                          uint32_t (*TE)[WT], uint32_t (*U)[NU], int* tie, int pp,
+                                          uint32_t lt, uint32_t eq, uint32_t z) {
   if (!lt) return;
   const uint32_t le = lt | eq;
   const uint32_t er = eq & ~z;
   if (!er) {
     lds_mark(A0[pp], le);
-  } else {
+  } else if constexpr (TIES) {
     lds_mark(A1[pp], le);
     lds_mark(TE[pp], er);
+  } else {
+    tie[pp] = 1;
+    return;
   }
   const uint32_t ez = eq & z;
   if (ez) {
@@ -690,7 +710,8 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
 constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2, kOrderSub = 3;
 // Default order of the single-instance path (measured: the subgroup steps
 // pay off up to d = 14; at d = 15, 16, with two points per workgroup, the
-// plain group rounds are faster).
+// plain group rounds are faster, unless the tie lattices are left out of
+// the workgroup, which frees the LDS the subgroup steps need).
 template <int D>
 constexpr int single_order() {
   return D <= 14 ? kOrderSub : kOrderGroup;
@@ -780,7 +801,13 @@ __device__ __forceinline__ void close_down(uint32_t (*A)[W]) {
 // call it.  order (uniform) and ix say how the candidates after the cover
 // set are visited (CandIndex).  Candidate ids are queued as id << 3 |
 // point, so m < 2^29.
-template <int D, int P, int NT, int ZB = zbits<D>()>
+//
+// With TIES false the workgroup holds A0 and U only (a third of the LDS at
+// d = 16, so more workgroups share a CU): a point whose exact tests meet a
+// residual tie (a pair with eR non-empty) is not stored but appended to
+// ix.defer, for a launch with TIES true.  Without such pairs A1 = A0 and TE
+// is empty, so the result is A0's.
+template <int D, int P, int NT, int ZB = zbits<D>(), bool TIES = true>
 __device__ __forceinline__ void member_body(const float* __restrict__ erows,
                                             const uint2* __restrict__ codes,
                                             const uint32_t* __restrict__ pos, uint32_t m,
@@ -792,10 +819,12 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   constexpr int QW = D == 16 ? 64 : 128;  // queue entries per wave
   constexpr int QCAP = (NT / 64) * QW;
   constexpr uint32_t FULL = (1u << D) - 1u;
+  constexpr int WT = TIES ? W : 1;  // the tie lattices' words
   __shared__ uint32_t A0[P][W];
-  __shared__ uint32_t A1[P][W];
-  __shared__ uint32_t TE[P][W];
+  __shared__ uint32_t A1[P][WT];
+  __shared__ uint32_t TE[P][WT];
   __shared__ uint32_t U[P][NU];
+  __shared__ int s_tie[P];  // a residual tie was met (TIES false)
   __shared__ float prow[P][D];
   __shared__ uint32_t queue[QCAP];
   // per-wave lists: the code-order groups' two lists (kList1 + kList2
@@ -845,13 +874,16 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     s_zm[tid] = z;
     s_rgt[tid] = g;
     s_any[tid] = 0;
+    s_tie[tid] = 0;
   }
   for (int k = tid; k < W; k += NT) {
 #pragma unroll
     for (int pp = 0; pp < P; ++pp) {
       A0[pp][k] = 0;
-      A1[pp][k] = 0;
-      TE[pp][k] = 0;
+      if constexpr (TIES) {
+        A1[pp][k] = 0;
+        TE[pp][k] = 0;
+      }
     }
   }
   for (int k = tid; k < NU; k += NT) {
@@ -874,7 +906,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   auto exact = [&](uint32_t b, int pp) {
     uint32_t lt, eq;
     pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
-    mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
+    mark_pair<TIES, W, NU, WT>(A0, A1, TE, U, s_tie, pp, lt, eq, s_zm[pp]);
   };
   // the queue is written and read by different lanes of one wave: LDS
   // operations of a wave complete in order, and the fences keep the
@@ -959,7 +991,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     const int pp = (int)(e - b * P);
     uint32_t lt, eq;
     pair_masks<D>(erows + (size_t)b * D, prow[pp], lt, eq);
-    mark_pair<W, NU>(A0, A1, TE, U, pp, lt, eq, s_zm[pp]);
+    mark_pair<TIES, W, NU, WT>(A0, A1, TE, U, s_tie, pp, lt, eq, s_zm[pp]);
   }
   __syncthreads();
   close_a0();
@@ -1251,6 +1283,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
     for (int pp = 0; pp < P; ++pp) {
       const uint32_t a0 = A0[pp][k];
       A0[pp][k] = low_closure(a0, D);
+      if constexpr (!TIES) continue;
       A1[pp][k] = low_closure(A1[pp][k] | a0, D);
       uint32_t t = TE[pp][k];
       const uint32_t z = __builtin_amdgcn_readfirstlane(s_zm[pp]);
@@ -1274,6 +1307,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 #pragma unroll
         for (int pp = 0; pp < P; ++pp) {
           A0[pp][lo] |= A0[pp][hi];
+          if constexpr (!TIES) continue;
           A1[pp][lo] |= A1[pp][hi];
           if (__builtin_amdgcn_readfirstlane(s_zm[pp]) >> (lv + 5) & 1u)
             TE[pp][hi] |= TE[pp][lo];
@@ -1308,7 +1342,9 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       const uint32_t r0 = ph == 0 ? m0 : 0u;
       const uint32_t r1 = ph == 0 ? m1 : (ph == 1 ? m0 : 0u);
       const uint32_t r2 = ~(r0 | r1);
-      const uint32_t a0 = A0[pp][k], a1 = A1[pp][k], t = TE[pp][k];
+      const uint32_t a0 = A0[pp][k];
+      const uint32_t a1 = TIES ? A1[pp][TIES ? k : 0] : a0;
+      const uint32_t t = TIES ? TE[pp][TIES ? k : 0] : 0u;
       const uint32_t sc = scope_mask(k, D, max_d);
       uint32_t w = ((a1 & ~t) | (a0 & t)) & r2;
       const uint32_t x = t & a1 & ~a0 & r2 & sc;
@@ -1320,9 +1356,13 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
         const uint32_t v = sz ? (U[pp][zcompress(sz, z)] >> j) & 1u : (rg >> j) & 1u;
         w |= v << b;
       }
-      A1[pp][k] = (w & sc) | ~sc;
-      TE[pp][k] = x;
-      if (x) s_any[pp] = 1;
+      if constexpr (TIES) {
+        A1[pp][k] = (w & sc) | ~sc;
+        TE[pp][k] = x;
+        if (x) s_any[pp] = 1;
+      } else {
+        A0[pp][k] = (w & sc) | ~sc;  // the same thread read it above
+      }
     }
   }
   __syncthreads();
@@ -1334,7 +1374,7 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   // candidate whose ub is not in that closure, or whose codes equal p's in
   // no column outside Z, is skipped before its row is read.
 #pragma unroll 1
-  for (int pp = 0; pp < P; ++pp) {
+  for (int pp = 0; pp < (TIES ? P : 0); ++pp) {
     if (!s_any[pp] || !((s_ok >> pp) & 1u)) continue;
     const uint32_t z = s_zm[pp];
     const uint2 pcp = codes[s_pt[pp]];
@@ -1416,16 +1456,21 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   }
   __syncthreads();
 
+  if constexpr (!TIES) {  // points that met a residual tie go to the full kernel
+    if (tid < P && s_tie[tid] && ((s_ok >> tid) & 1u))
+      ix.defer[1 + atomicAdd(ix.defer, 1u)] = s_pt[tid];
+  }
 #pragma unroll
   for (int pp = 0; pp < P; ++pp) {
-    if (!((s_ok >> pp) & 1u)) continue;
+    if (!((s_ok >> pp) & 1u) || (!TIES && s_tie[pp])) continue;
+    uint32_t(*R)[W] = TIES ? reinterpret_cast<uint32_t(*)[W]>(&A1[0][0]) : A0;  // the final words
     const uint32_t row = pos[s_pt[pp]];
     // streaming stores: the bitmaps are not read again here, and keeping
     // them out of L2 leaves it to the candidate codes the other workgroups
     // are scanning
     uint32_t cnt = 0;
     for (int k = tid; k < W; k += NT) {
-      const uint32_t v = A1[pp][k];
+      const uint32_t v = R[pp][k];
       cnt += v != 0xFFFFFFFFu;
       __builtin_nontemporal_store(v, &dom[(size_t)row * W + k]);
     }
@@ -1442,15 +1487,21 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 // narrow window of the code order, spread over the eight XCDs.  (Giving each
 // XCD a contiguous run of tiles instead was measured slower: the cost of a
 // tile varies along the code order, so the XCDs finish unevenly.)
-template <int D, int P, int NT, int ORDER, int ZB = zbits<D>()>
-__global__ __launch_bounds__(NT) void k_member(const float* __restrict__ erows,
-                                               const uint2* __restrict__ codes,
-                                               const uint32_t* __restrict__ pos, uint32_t m,
-                                               int max_d, const uint32_t* __restrict__ minkey,
-                                               uint32_t* __restrict__ dom, uint32_t cover,
-                                               CandIndex ix) {
-  member_body<D, P, NT, ZB>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P, cover,
-                            ORDER, ix);
+//
+// TIES false (tie lattices left out, ix.defer required): at d = 16 eight
+// waves per SIMD, four workgroups per CU.
+template <int D, bool TIES>
+constexpr int member_waves() {
+  return !TIES && D == 16 ? 8 : 1;
+}
+template <int D, int P, int NT, int ORDER, int ZB = zbits<D>(), bool TIES = true>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(member_waves<D, TIES>())))
+void k_member(const float* __restrict__ erows, const uint2* __restrict__ codes,
+              const uint32_t* __restrict__ pos, uint32_t m, int max_d,
+              const uint32_t* __restrict__ minkey, uint32_t* __restrict__ dom, uint32_t cover,
+              CandIndex ix) {
+  member_body<D, P, NT, ZB, TIES>(erows, codes, pos, m, max_d, minkey, dom, blockIdx.x * P,
+                                  cover, ORDER, ix);
 }
 
 // The points a k_member launch with a narrower boundary table deferred:

@@ -102,20 +102,27 @@ int member_groups(sky_ctx* c, uint32_t ne) {
 }
 
 // The membership launch of the single-instance path.  Where the kernel runs
-// with a narrower boundary table (d >= 15), the points it defers follow in a
-// second launch with the full table.
+// with a narrower boundary table (d = 16), the points it defers follow in a
+// second launch with the full table; there, unless the data has ties
+// (ties), the first launch also leaves out the tie lattices and defers the
+// points that meet a residual tie.
 template <int D, int PP, int NT, int O>
 int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minkey,
-                  uint32_t cover, sky::CandIndex ix) {
+                  uint32_t cover, sky::CandIndex ix, bool ties) {
   const dim3 grid((ne + PP - 1) / PP), blk(NT);
   constexpr int ZN = sky::narrow_zbits<D>();
   if constexpr (ZN < sky::zbits<D>()) {
     TRY(ensure(c, c->defer, sizeof(uint32_t) * ((size_t)ne + 1)));
     TRY(fill32(c, c->defer.p, 1, 0u));
     ix.defer = P<uint32_t>(c->defer);
-    hipLaunchKernelGGL((sky::k_member<D, PP, NT, O, ZN>), grid, blk, 0, c->stream,
-                       P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                       (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+    if (ties)
+      hipLaunchKernelGGL((sky::k_member<D, PP, NT, O, ZN>), grid, blk, 0, c->stream,
+                         P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                         (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
+    else
+      hipLaunchKernelGGL((sky::k_member<D, PP, NT, O, ZN, false>), grid, blk, 0, c->stream,
+                         P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                         (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
     LAUNCHCHK();
     ix.defer = nullptr;
     // one point per workgroup: the deferred points are few, and their
@@ -193,8 +200,9 @@ int split_d(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, sky:
                      P<uint32_t>(c->s1pid));
   LAUNCHCHK();
   TRY(mark(c, SKY_STAGE_SELF));
+  // counter[2]: tied sample values of the survivors' columns
   hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D * sky::kPivotSplit), dim3(kBlock), 0, c->stream,
-                     P<float>(c->s1rows), m, P<uint32_t>(c->piv));
+                     P<float>(c->s1rows), m, P<uint32_t>(c->piv), P<uint32_t>(c->counter) + 2);
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_codes<D>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
                      P<float>(c->s1rows), m, P<uint32_t>(c->piv), P<uint2>(c->codes),
@@ -220,12 +228,12 @@ int split_d(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, sky:
 }
 
 template <int D>
-int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
+int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey, bool ties) {
   hipLaunchKernelGGL(sky::k_colmin<D>, dim3(std::min<unsigned>(grid_for(ne), 1024u)), dim3(kBlock),
                      0, c->stream, P<float>(c->erows), ne, minkey);
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_pivots<D>, dim3(D * sky::kPivotSplit), dim3(kBlock), 0, c->stream,
-                     P<float>(c->erows), ne, P<uint32_t>(c->piv));
+                     P<float>(c->erows), ne, P<uint32_t>(c->piv), nullptr);
   LAUNCHCHK();
   hipLaunchKernelGGL(sky::k_codes<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
                      P<float>(c->erows), ne, P<uint32_t>(c->piv), P<uint2>(c->codes),
@@ -237,7 +245,8 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
   constexpr int NT = sky::member_threads<D>();
   const Tuning& tu = c->tune[D];
   int order = c->opt_member_index;
-  if (order < 0) order = sky::single_order<D>();
+  constexpr bool narrow = sky::narrow_zbits<D>() < sky::zbits<D>();
+  if (order < 0) order = narrow && !ties ? sky::kOrderSub : sky::single_order<D>();
   const uint32_t cover = tu.cover ? tu.cover : sky::auto_cover(ne);
   sky::CandIndex ix;
   ix.head = tu.head;
@@ -264,16 +273,16 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey) {
   }
   TRY(mark(c, SKY_STAGE_MEMBER));  // the membership kernel launches alone
   if (order == sky::kOrderSub)
-    return launch_member<D, PP, NT, sky::kOrderSub>(c, ne, max_d, minkey, cover, ix);
+    return launch_member<D, PP, NT, sky::kOrderSub>(c, ne, max_d, minkey, cover, ix, ties);
   if (order == sky::kOrderGroup)
-    return launch_member<D, PP, NT, sky::kOrderGroup>(c, ne, max_d, minkey, cover, ix);
+    return launch_member<D, PP, NT, sky::kOrderGroup>(c, ne, max_d, minkey, cover, ix, ties);
   if (order == sky::kOrderBucket)
-    return launch_member<D, PP, NT, sky::kOrderBucket>(c, ne, max_d, minkey, cover, ix);
-  return launch_member<D, PP, NT, sky::kOrderScan>(c, ne, max_d, minkey, cover, ix);
+    return launch_member<D, PP, NT, sky::kOrderBucket>(c, ne, max_d, minkey, cover, ix, ties);
+  return launch_member<D, PP, NT, sky::kOrderScan>(c, ne, max_d, minkey, cover, ix, ties);
 }
 
 template int split_d<SKY_D>(sky_ctx*, const float*, uint32_t, uint32_t, sky::DimList,
                             const uint32_t*, uint32_t, int, uint32_t*);
-template int member_d<SKY_D>(sky_ctx*, uint32_t, uint32_t, uint32_t*);
+template int member_d<SKY_D>(sky_ctx*, uint32_t, uint32_t, uint32_t*, bool);
 
 }  // namespace skyi

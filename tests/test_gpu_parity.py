@@ -40,6 +40,25 @@ def test_mdmc_every_d_ties(ctx, d):
     check_mdmc(ctx, npref.tie_heavy(rng, n, d, 3), tag="ties")
 
 
+@pytest.mark.parametrize("order,max_d", [(-1, None), (2, 9), (3, None)])
+def test_mdmc_d16_rare_ties(order, max_d):
+    """d = 16 data that is tie-free but for a few planted pairs: the
+    membership runs without its tie lattices, and the points whose exact
+    tests meet a residual tie (q equal to p in one column, below it in the
+    rest) go to the full kernel."""
+    rng = np.random.default_rng(2600 + order)
+    x = rng.random((2500, 16), dtype=np.float32) * np.float32(0.5) + np.float32(0.25)
+    p = rng.choice(2500, size=80, replace=False)
+    q = x[p] - np.float32(1e-3)
+    cols = rng.integers(1, 16, size=80)
+    q[np.arange(80), cols] = x[p, cols]  # tied in one column ...
+    q[:, 0] = x[p, 0]  # ... and in column 0
+    x = np.concatenate([x, q]).astype(np.float32)
+    with skycube_amd.Context(0) as c:
+        c.set_option(skycube_amd.SKY_OPT_MEMBER_INDEX, order)
+        check_mdmc(c, x, max_d, tag=f"rare ties order {order}")
+
+
 @pytest.mark.parametrize("d", [2, 5, 8, 11, 14, 16])
 def test_mdmc_every_d_continuous(ctx, d):
     rng = np.random.default_rng(2000 + d)
