@@ -90,7 +90,7 @@ void sky_host_free(void *ptr);
  *                         [2048, 16384], per instance in a batch)
  *   SKY_OPT_COVER_HEAD    how many of them are tested pair by pair before
  *                         the rest of the cover is code-tested against the
- *                         closure of the head (0: a quarter of the cover)
+ *                         closure of the head (0: an eighth of the cover)
  *   SKY_OPT_PRUNERS       lowest-norm points of the prefilter (128; 64 .. 8192)
  *   SKY_OPT_MIN_BUCKET    bucket-size threshold above (16; 1 .. 2^30)
  *   SKY_OPT_MEMBER_GRID   workgroups of the batched persistent membership

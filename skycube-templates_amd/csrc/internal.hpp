@@ -71,7 +71,7 @@ struct Tuning {
   uint32_t grid = 2048;                   // workgroups of the persistent kernel
   uint32_t tile = 0;                      // 0: member_points, 1: half of it
   uint32_t order = 3;                     // batched membership: 3 subgroups, 2 groups, 0 scan / buckets
-  uint32_t head = 0;                      // cover head tested pair by pair (0: cover / 4)
+  uint32_t head = 0;                      // cover head tested pair by pair (0: cover / 8)
 };
 
 // Bump allocator over the allocation-free path's scratch array.

@@ -717,7 +717,7 @@ constexpr int single_order() {
   return D <= 14 ? kOrderSub : kOrderGroup;
 }
 struct CandIndex {
-  uint32_t head = 0;  // cover candidates tested pair by pair (0: a quarter of the cover)
+  uint32_t head = 0;  // cover candidates tested pair by pair (0: an eighth of the cover)
   const uint32_t* bmk = nullptr;
   const uint32_t* bstart = nullptr;
   const uint32_t* bcnt = nullptr;
@@ -981,11 +981,11 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
   };
   auto close_a0 = [&]() { close_down<D, P, W, NT>(A0); };
 
-  // pass 1a: the cover.  Its head (a quarter by default) is tested pair by pair; A0 is
+  // pass 1a: the cover.  Its head (an eighth by default) is tested pair by pair; A0 is
   // closed; the rest of the cover is code-tested against that A0 (in
   // score order, the likely dominators first); A0 is closed again.
   const uint32_t kc = min(m, cover);
-  const uint32_t kc0 = min(kc, ix.head ? ix.head : max(64u, cover / 4));
+  const uint32_t kc0 = min(kc, ix.head ? ix.head : max(64u, cover / 8));
   for (uint32_t e = tid; e < kc0 * P; e += NT) {
     const uint32_t b = e / P;
     const int pp = (int)(e - b * P);
