@@ -479,7 +479,8 @@ def main() -> int:
         "window_parts_ms": win["parts_ms"],
         "stages_ms": dict(st, device_total=round(dev_ms, 3)),
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_member<d={d}> (code-order {'groups' if d >= 15 else 'subgroups'})",
+                     "kernel": (f"k_member<d={d}>: the membership stage (its main launch and, "
+                                "at d = 16, the short launch of its deferred points)"),
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": B},
