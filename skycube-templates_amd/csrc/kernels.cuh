@@ -709,12 +709,12 @@ __device__ __forceinline__ void pair_masks(const float* __restrict__ q, const fl
 //                 is tested
 constexpr int kOrderScan = 0, kOrderBucket = 1, kOrderGroup = 2, kOrderSub = 3;
 // Default order of the single-instance path (measured: the subgroup steps
-// pay off up to d = 14; at d = 15, 16, with two points per workgroup, the
-// plain group rounds are faster, unless the tie lattices are left out of
-// the workgroup, which frees the LDS the subgroup steps need).
+// pay off up to d = 14 and at d = 16, where the launch keeps four
+// workgroups per CU (one point each with the tie lattices, two without);
+// at d = 15 the two orders are level).
 template <int D>
 constexpr int single_order() {
-  return D <= 14 ? kOrderSub : kOrderGroup;
+  return D == 15 ? kOrderGroup : kOrderSub;
 }
 struct CandIndex {
   uint32_t head = 0;  // cover candidates tested pair by pair (0: an eighth of the cover)
@@ -1488,11 +1488,11 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
 // XCD a contiguous run of tiles instead was measured slower: the cost of a
 // tile varies along the code order, so the XCDs finish unevenly.)
 //
-// TIES false (tie lattices left out, ix.defer required): at d = 16 eight
-// waves per SIMD, four workgroups per CU.
+// At d = 16 (one point per workgroup with the tie lattices, two without
+// them) the kernel is held to eight waves per SIMD, four workgroups per CU.
 template <int D, bool TIES>
 constexpr int member_waves() {
-  return !TIES && D == 16 ? 8 : 1;
+  return D == 16 ? 8 : 1;
 }
 template <int D, int P, int NT, int ORDER, int ZB = zbits<D>(), bool TIES = true>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(member_waves<D, TIES>())))

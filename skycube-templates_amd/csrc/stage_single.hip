@@ -105,7 +105,10 @@ int member_groups(sky_ctx* c, uint32_t ne) {
 // with a narrower boundary table (d = 16), the points it defers follow in a
 // second launch with the full table; there, unless the data has ties
 // (ties), the first launch also leaves out the tie lattices and defers the
-// points that meet a residual tie.
+// points that meet a residual tie.  With the tie lattices, one point per
+// workgroup (24 KB of lattices, four workgroups per CU) measured faster at
+// d = 16 than two (three workgroups per CU): 7.9 vs 8.8 ms on the
+// integer-valued n = 1M workload.
 template <int D, int PP, int NT, int O>
 int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minkey,
                   uint32_t cover, sky::CandIndex ix, bool ties) {
@@ -116,7 +119,7 @@ int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minke
     TRY(fill32(c, c->defer.p, 1, 0u));
     ix.defer = P<uint32_t>(c->defer);
     if (ties)
-      hipLaunchKernelGGL((sky::k_member<D, PP, NT, O, ZN>), grid, blk, 0, c->stream,
+      hipLaunchKernelGGL((sky::k_member<D, 1, NT, O, ZN>), dim3(ne), blk, 0, c->stream,
                          P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
                          (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
     else
@@ -245,8 +248,7 @@ int member_d(sky_ctx* c, uint32_t ne, uint32_t max_d, uint32_t* minkey, bool tie
   constexpr int NT = sky::member_threads<D>();
   const Tuning& tu = c->tune[D];
   int order = c->opt_member_index;
-  constexpr bool narrow = sky::narrow_zbits<D>() < sky::zbits<D>();
-  if (order < 0) order = narrow && !ties ? sky::kOrderSub : sky::single_order<D>();
+  if (order < 0) order = sky::single_order<D>();
   const uint32_t cover = tu.cover ? tu.cover : sky::auto_cover(ne);
   sky::CandIndex ix;
   ix.head = tu.head;
