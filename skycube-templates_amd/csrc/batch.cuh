@@ -532,13 +532,13 @@ __global__ void k_btasks(const uint32_t* __restrict__ off, uint32_t T, uint32_t 
 // (bmember_waves; not from d = 11 on, where the spills grow past what the
 // occupancy gains, nor for the subgroup steps with eight points per
 // workgroup from d = 8 on, which need more registers).  With eight waves per
-// workgroup (d = 13, 14) the kernel is held to 80 VGPRs instead, six waves
-// per SIMD, as many as its LDS allows: a large instance then runs as fast
-// as in the single-instance kernel (which needs 79).
+// workgroup the kernel is held to 64 VGPRs at d = 13 (eight waves per SIMD;
+// a few values spill, and a large instance still runs 8-9% faster) and to
+// 80 at d = 14, six waves per SIMD, as many as its LDS allows.
 template <int D, int P, int NT, int ORD>
 constexpr int bmember_waves() {
   return ORD != 0 && NT == 256 && D <= 10 && !(ORD == kOrderSub && P == 8 && D >= 8) ? 8
-         : (ORD != 0 && NT == 512 && D <= 14 ? 6 : 1);
+         : (ORD != 0 && NT == 512 && D == 13 ? 8 : (ORD != 0 && NT == 512 && D <= 14 ? 6 : 1));
 }
 
 template <int D, int P, int NT, int ORD = 0>
