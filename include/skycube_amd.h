@@ -201,7 +201,10 @@ int sky_mdmc_device_results(sky_ctx *ctx, const uint32_t **ext_pids_dev,
  * ascending by (j, v): group_word[g], group_value[g], and the ids
  * pids[group_off[g] .. group_off[g+1]) ascending.  Call with NULL arrays (or
  * too small capacities) to learn *n_groups and *n_entries first; the grouping
- * is kept on the device until the next call.
+ * is kept on the device until the next call.  Arrays given to the first call
+ * with room enough receive the ids while the groups are still being formed
+ * (the copy overlaps the grouping); with too little room the call fails with
+ * SKY_ERR_CAPACITY, and the sizes are set.
  */
 int sky_mdmc_hashcube(sky_ctx *ctx, uint64_t *n_groups, uint64_t *n_entries,
                       uint32_t *group_word, uint32_t *group_value,
