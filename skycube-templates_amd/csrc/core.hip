@@ -147,6 +147,14 @@ int fill32(sky_ctx* c, void* p, uint64_t n, uint32_t v) {
   return SKY_OK;
 }
 
+int side_stream(sky_ctx* c) {
+  if (!c->copy_stream)
+    HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  if (!c->sorted_ev) HIPCHK(hipEventCreateWithFlags(&c->sorted_ev, hipEventDisableTiming));
+  if (!c->side_ev) HIPCHK(hipEventCreateWithFlags(&c->side_ev, hipEventDisableTiming));
+  return SKY_OK;
+}
+
 int fill8(sky_ctx* c, void* p, uint64_t n, uint8_t v) {
   if (!n) return SKY_OK;
   hipLaunchKernelGGL(sky::k_fill_u8, dim3((unsigned)std::min<uint64_t>(grid_for(n), 4096)),
@@ -443,10 +451,7 @@ int build_hashcube_impl(sky_ctx* c, uint32_t* pids, uint64_t cap_entries, bool* 
     TRY(sort_pairs(c, P<uint64_t>(c->hc_key_a), P<uint64_t>(c->hc_key_b),
                    P<uint32_t>(c->hc_val_a), P<uint32_t>(c->hc_val_b), total, 32 + wbits));
     if (pids && cap_entries >= (uint64_t)total + (extra ? ne : 0)) {
-      if (!c->copy_stream) {
-        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c->sorted_ev, hipEventDisableTiming));
-      }
+      TRY(side_stream(c));
       HIPCHK(hipEventRecord(c->sorted_ev, c->stream));
       HIPCHK(hipStreamWaitEvent(c->copy_stream, c->sorted_ev, 0));
       *early = true;
@@ -793,6 +798,7 @@ void sky_ctx_destroy(sky_ctx* c) {
     (void)hipStreamDestroy(c->copy_stream);
   }
   if (c->sorted_ev) (void)hipEventDestroy(c->sorted_ev);
+  if (c->side_ev) (void)hipEventDestroy(c->side_ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -106,9 +106,10 @@ struct sky_ctx {
   // grouping of the last HashCube export
   bool has_hc = false;
   uint64_t hc_groups = 0, hc_entries = 0, hc_sorted = 0;
-  // the export's early copy of the ids (created on first use)
+  // a second stream (created on first use, side_stream): the export's early
+  // copy of the ids, the membership's deferred points
   hipStream_t copy_stream = nullptr;
-  hipEvent_t sorted_ev = nullptr;
+  hipEvent_t sorted_ev = nullptr, side_ev = nullptr;
   // stage timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[SKY_STAGE_COUNT + 1] = {};
@@ -217,6 +218,8 @@ int sort_pairs(sky_ctx* c, const uint64_t* kin, uint64_t* kout, const uint32_t* 
 
 // Fills on the context stream by kernels (no memset nodes in a captured graph).
 int fill32(sky_ctx* c, void* p, uint64_t n, uint32_t v);
+// Creates the context's second stream and its events if not yet there.
+int side_stream(sky_ctx* c);
 int fill8(sky_ctx* c, void* p, uint64_t n, uint8_t v);
 
 inline sky::DimList dims_of(uint32_t sub, uint32_t* ds) {

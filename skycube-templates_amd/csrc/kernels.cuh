@@ -734,7 +734,11 @@ struct CandIndex {
   uint32_t npts = 0;
   // if set, a point with more minimum columns than the kernel's boundary
   // table holds is not computed but appended to defer[1 ..] (count defer[0])
+  // (with the tie lattices left out, so is a point that meets a residual tie)
   uint32_t* defer = nullptr;
+  // if set, such a point is skipped without being appended: k_defer_list
+  // listed it before the launch
+  bool defer_listed = false;
   // if set, rowcnt[row] gains the number of the row's words that are not
   // all ones (the HashCube export's entries of that point)
   uint32_t* rowcnt = nullptr;
@@ -864,8 +868,8 @@ __device__ __forceinline__ void member_body(const float* __restrict__ erows,
       }
       g |= (uint32_t)(!at_min) << k;
     }
-    if (ix.defer && ok && __popc(~g & FULL) > ZB) {
-      ix.defer[1 + atomicAdd(ix.defer, 1u)] = a;
+    if ((ix.defer || ix.defer_listed) && ok && __popc(~g & FULL) > ZB) {
+      if (!ix.defer_listed) ix.defer[1 + atomicAdd(ix.defer, 1u)] = a;
       ok = false;
     }
     const uint32_t okm = (uint32_t)__ballot(ok);  // the P lanes are in wave 0
@@ -1506,6 +1510,22 @@ void k_member(const float* __restrict__ erows, const uint2* __restrict__ codes,
                                   cover, ORDER, ix);
 }
 
+// The points (score-order positions, m of them) with more than zb columns at
+// the column minimum, appended to list[1 ..] (count list[0], which starts at
+// zero): the points a k_member launch with a 2^zb-word boundary table
+// leaves to k_member_deferred (the same test as member_body's).
+template <int D>
+__global__ void k_defer_list(const float* __restrict__ erows, uint32_t m,
+                             const uint32_t* __restrict__ minkey, int zb,
+                             uint32_t* __restrict__ list) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= m) return;
+  int nz = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) nz += order_key(erows[(size_t)a * D + k]) == minkey[k];
+  if (nz > zb) list[1 + atomicAdd(list, 1u)] = a;
+}
+
 // The points a k_member launch with a narrower boundary table deferred:
 // list[0] of them at list[1 ..], P per workgroup, the grid looping over the
 // list (its length is known on the device only).
@@ -1518,6 +1538,7 @@ __global__ __launch_bounds__(NT) void k_member_deferred(
   ix.pts = list + 1;
   ix.npts = n;
   ix.defer = nullptr;
+  ix.defer_listed = false;
   for (uint32_t base = blockIdx.x * P; base < n; base += gridDim.x * P)
     member_body<D, P, NT>(erows, codes, pos, m, max_d, minkey, dom, base, cover, ORDER, ix);
 }

@@ -108,16 +108,45 @@ int member_groups(sky_ctx* c, uint32_t ne) {
 // points that meet a residual tie.  With the tie lattices, one point per
 // workgroup (24 KB of lattices, four workgroups per CU) measured faster at
 // d = 16 than two (three workgroups per CU): 7.9 vs 8.8 ms on the
-// integer-valued n = 1M workload.
+// integer-valued n = 1M workload.  The points beyond the narrow table are
+// listed before the launch (k_defer_list) and computed on the context's
+// second stream beside it, where their few long workgroups hide in the
+// main launch; the allocation-free call keeps to one stream and lets the
+// kernel list them.
 template <int D, int PP, int NT, int O>
 int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minkey,
                   uint32_t cover, sky::CandIndex ix, bool ties) {
   const dim3 grid((ne + PP - 1) / PP), blk(NT);
   constexpr int ZN = sky::narrow_zbits<D>();
   if constexpr (ZN < sky::zbits<D>()) {
-    TRY(ensure(c, c->defer, sizeof(uint32_t) * ((size_t)ne + 1)));
-    TRY(fill32(c, c->defer.p, 1, 0u));
-    ix.defer = P<uint32_t>(c->defer);
+    // one point per workgroup for the deferred points: they are few
+    const dim3 grid2(std::min<uint32_t>(ne, 512u));
+    const bool side = !c->carve;
+    const size_t L = (size_t)ne + 1;
+    // lists: [0, L) the points beyond the table (allocation-free: every
+    // deferred point), [L, 2L) the points that met a residual tie
+    TRY(ensure(c, c->defer, sizeof(uint32_t) * (side ? 2 * L : L)));
+    uint32_t* pre = P<uint32_t>(c->defer);
+    uint32_t* tie = side ? pre + L : pre;
+    TRY(fill32(c, pre, 1, 0u));
+    if (side) {
+      TRY(fill32(c, tie, 1, 0u));
+      TRY(side_stream(c));
+      hipLaunchKernelGGL(sky::k_defer_list<D>, dim3(grid_for(ne)), dim3(kBlock), 0, c->stream,
+                         P<float>(c->erows), ne, minkey, ZN, pre);
+      LAUNCHCHK();
+      HIPCHK(hipEventRecord(c->side_ev, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->copy_stream, c->side_ev, 0));
+      hipLaunchKernelGGL((sky::k_member_deferred<D, 1, NT, O>), grid2, blk, 0, c->copy_stream,
+                         P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                         (int)max_d, minkey, P<uint32_t>(c->dom), cover, pre, ix);
+      LAUNCHCHK();
+      HIPCHK(hipEventRecord(c->side_ev, c->copy_stream));
+      ix.defer_listed = true;
+      ix.defer = ties ? nullptr : tie;
+    } else {
+      ix.defer = pre;
+    }
     if (ties)
       hipLaunchKernelGGL((sky::k_member<D, 1, NT, O, ZN>), dim3(ne), blk, 0, c->stream,
                          P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
@@ -128,13 +157,14 @@ int launch_member(sky_ctx* c, uint32_t ne, uint32_t max_d, const uint32_t* minke
                          (int)max_d, minkey, P<uint32_t>(c->dom), cover, ix);
     LAUNCHCHK();
     ix.defer = nullptr;
-    // one point per workgroup: the deferred points are few, and their
-    // workgroups are the tail of the pass
-    const dim3 grid2(std::min<uint32_t>(ne, 512u));
-    hipLaunchKernelGGL((sky::k_member_deferred<D, 1, NT, O>), grid2, blk, 0, c->stream,
-                       P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
-                       (int)max_d, minkey, P<uint32_t>(c->dom), cover, P<uint32_t>(c->defer), ix);
-    LAUNCHCHK();
+    ix.defer_listed = false;
+    if (!side || !ties) {  // the points the launch deferred, after it
+      hipLaunchKernelGGL((sky::k_member_deferred<D, 1, NT, O>), grid2, blk, 0, c->stream,
+                         P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
+                         (int)max_d, minkey, P<uint32_t>(c->dom), cover, tie, ix);
+      LAUNCHCHK();
+    }
+    if (side) HIPCHK(hipStreamWaitEvent(c->stream, c->side_ev, 0));
   } else {
     hipLaunchKernelGGL((sky::k_member<D, PP, NT, O>), grid, blk, 0, c->stream,
                        P<float>(c->erows), P<uint2>(c->codes), P<uint32_t>(c->pos), ne,
