@@ -89,6 +89,19 @@ def test_mdmc_d16_many_minimum_columns(ctx, max_d):
     check_mdmc(ctx, x, max_d, tag="d16-minima")
 
 
+@pytest.mark.parametrize("max_d", [16, 8])
+def test_mdmc_d16_tie_free_with_minima(ctx, max_d):
+    """d = 16, continuous values clipped at 0: the only ties are at the
+    column minima (boundary columns), so the launch leaves out its tie
+    lattices, while the many points with more than 8 columns at 0 are listed
+    before it and computed beside it with the full boundary table."""
+    rng = np.random.default_rng(16017)
+    x = np.maximum(rng.standard_normal((1500, 16)).astype(np.float32), np.float32(0.0))
+    at_min = (x == 0).sum(axis=1)
+    assert (at_min > 8).sum() > 100 and (at_min <= 8).sum() > 100
+    check_mdmc(ctx, x, max_d, tag="d16-tie-free-minima")
+
+
 @pytest.mark.parametrize("d,max_d", [(6, 1), (9, 3), (12, 5), (16, 4)])
 def test_mdmc_max_d(ctx, d, max_d):
     rng = np.random.default_rng(3000 + d)
