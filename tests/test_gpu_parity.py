@@ -3,6 +3,8 @@
 Bit-exact: the ext pids and every membership word must be identical (bits of
 out-of-scope subspaces are 1 on both sides).
 """
+import json
+import os
 import sys
 
 import numpy as np
@@ -260,15 +262,18 @@ def test_mdmc_member_candidate_order(d, mode):
 
 
 def test_mdmc_independent_16d_sampled(ctx):
-    """n = 1M, d = 16 independent data (635,442 extended-skyline points):
-    too large for a full oracle run, so checked by sampling: sampled points
-    of the extended skyline have no strict dominator and their bitmaps equal
-    the oracle's; sampled points outside it have one."""
+    """n = 1M, d = 16 independent data: the extended skyline's size is the
+    oracle's (its full run is in the golden digests, checked word for word
+    by test_golden.py), and sampled points are checked here directly:
+    sampled points of the extended skyline have no strict dominator and
+    their bitmaps equal the oracle's; sampled points outside it have one."""
     w = datagen.WORKLOADS["independent_16d"]
     x = datagen.workload("independent_16d")
     _say(f"independent_16d n={x.shape[0]} d={x.shape[1]} max_d={w['max_d']}")
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "golden.json")))["full"]["independent_16d"]
     ext, dom = ctx.mdmc(x, w["max_d"])
-    assert ext.size == 635442
+    assert ext.size == gold["n_ext"]
     rng = np.random.default_rng(7)
     smp = np.sort(rng.choice(ext.size, size=48, replace=False))
     odom = skyorc.mdmc_sample(x, ext, smp, w["max_d"])
@@ -342,8 +347,6 @@ def test_sdsc_anticorrelated_full_is_golden(ctx):
     and the skylines, turned into per-point bitmaps, hash to the golden
     digest of the oracle's MDMC result."""
     import hashlib
-    import json
-    import os
     gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
                                        "golden.json")))["full"]["anticorrelated_plane"]
     x = datagen.workload("anticorrelated_plane")
