@@ -341,25 +341,32 @@ __global__ void k_bidx_first(const uint32_t* __restrict__ brun, const uint32_t* 
 // against the survivors of its own instance (self_idx, shared with the
 // single-instance kernel; boff is the instance-major bucket table).  With
 // need_dom, dominated (1) and undominated (0) survivors are told apart and
-// minkey is not used.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_bself_idx(
+// minkey is not used.  NT threads per workgroup; the waves stride over the
+// survivors when the grid holds fewer.  (The SDSC levels run it with one
+// wave per workgroup and per survivor: 12% faster there, where a finished
+// wave's slot is taken at once by the next survivor; the batched MDMC pass
+// with four waves, its grid then needing a quarter of the workgroups.)
+template <int D, int NT = kBlock>
+__global__ __launch_bounds__(NT) void k_bself_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
     const uint2* __restrict__ codes, const uint32_t* __restrict__ off, uint32_t T,
     const uint32_t* __restrict__ count, const uint32_t* __restrict__ boff,
     const uint32_t* __restrict__ sval, const uint32_t* __restrict__ bkey,
     const uint2* __restrict__ bcode, const uint32_t* __restrict__ minkey, int need_dom,
     uint8_t* __restrict__ status) {
-  __shared__ uint32_t sLo[kBlock / 64][64];
-  __shared__ uint32_t sEx[kBlock / 64][64];
+  __shared__ uint32_t sLo[NT / 64][64];
+  __shared__ uint32_t sEx[NT / 64][64];
   const int wv = threadIdx.x >> 6;
-  const uint32_t j = blockIdx.x * (kBlock / 64) + wv;
-  if (j >= *count) return;  // whole wave; no block barrier in this kernel
-  const uint32_t t = seg_of(off, T, j);
-  const uint8_t st = self_idx<D>(rows, keys, codes, boff + ((size_t)t << D), sval, bkey, bcode, j,
-                                 need_dom, need_dom ? nullptr : minkey + (size_t)t * 16, sLo[wv],
-                                 sEx[wv]);
-  if ((threadIdx.x & 63) == 0) status[j] = st;
+  const uint32_t cnt = *count;
+  const uint32_t stride = gridDim.x * (uint32_t)(NT / 64);
+  // whole waves; no block barrier in this kernel
+  for (uint32_t j = blockIdx.x * (uint32_t)(NT / 64) + (uint32_t)wv; j < cnt; j += stride) {
+    const uint32_t t = seg_of(off, T, j);
+    const uint8_t st = self_idx<D>(rows, keys, codes, boff + ((size_t)t << D), sval, bkey, bcode,
+                                   j, need_dom, need_dom ? nullptr : minkey + (size_t)t * 16,
+                                   sLo[wv], sEx[wv]);
+    if ((threadIdx.x & 63) == 0) status[j] = st;
+  }
 }
 
 // SDSC chunk: the rows of candidate i of instance t (cuboid subs[t]),

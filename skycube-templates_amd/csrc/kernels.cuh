@@ -399,21 +399,22 @@ __device__ __forceinline__ uint8_t self_idx(const float* __restrict__ rows,
   return st;
 }
 
-// Held to 64 VGPRs (eight waves per SIMD).
+// One survivor per workgroup of one wave (measured 16-20% faster than four
+// per workgroup of four waves: a finished wave's slot is taken at once by
+// the next survivor), held to 64 VGPRs (eight waves per SIMD).
 template <int D>
-__global__ __attribute__((amdgpu_waves_per_eu(8))) __launch_bounds__(kBlock) void k_self_idx(
+__global__ __attribute__((amdgpu_waves_per_eu(8))) __launch_bounds__(64) void k_self_idx(
     const float* __restrict__ rows, const uint32_t* __restrict__ keys,
     const uint2* __restrict__ codes,
     const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bidx,
     const uint32_t* __restrict__ bkey, const uint2* __restrict__ bcode, uint32_t m, int need_dom,
     const uint32_t* __restrict__ minkey, uint8_t* __restrict__ status) {
-  __shared__ uint32_t sLo[kBlock / 64][64];
-  __shared__ uint32_t sEx[kBlock / 64][64];
-  const int wv = threadIdx.x >> 6;
-  const uint32_t i = blockIdx.x * (kBlock / 64) + wv;
-  if (i >= m) return;  // whole wave; the kernel has no block barrier
+  __shared__ uint32_t sLo[64];
+  __shared__ uint32_t sEx[64];
+  const uint32_t i = blockIdx.x;
+  if (i >= m) return;
   const uint8_t st = self_idx<D>(rows, keys, codes, boff, bidx, bkey, bcode, i, need_dom,
-                                 minkey, sLo[wv], sEx[wv]);
+                                 minkey, sLo, sEx);
   if ((threadIdx.x & 63) == 0) status[i] = st;
 }
 

@@ -281,10 +281,15 @@ int sdsc_chunk_d(sky_ctx* c, const SdscChunk& k) {
                        P<uint32_t>(B.isort), P<uint32_t>(B.s1keys), P<uint2>(B.codes), cnt + 1,
                        nullptr, T, P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr);
     LAUNCHCHK();
-    hipLaunchKernelGGL(sky::k_bself_idx<D>, dim3((N + 3) / 4), dim3(kBlock), 0, c->stream,
-                       P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes), s1off, T,
-                       cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort), P<uint32_t>(B.ibkey),
-                       P<uint2>(B.ibcode), nullptr, 1, P<uint8_t>(B.status));
+    // one wave per workgroup and per survivor (the dispatcher balances the
+    // uneven survivors; a grid with fewer workgroups that stride measured
+    // 15% slower), up to 2^26 - 1 workgroups, beyond which the waves stride
+    hipLaunchKernelGGL((sky::k_bself_idx<D, 64>), dim3(std::min<uint32_t>(N, (1u << 26) - 1u)),
+                       dim3(64),
+                       0, c->stream, P<float>(B.s1rows), P<uint32_t>(B.s1keys), P<uint2>(B.codes),
+                       s1off, T, cnt + 1, P<uint32_t>(B.iboff), P<uint32_t>(B.isort),
+                       P<uint32_t>(B.ibkey), P<uint2>(B.ibcode), nullptr, 1,
+                       P<uint8_t>(B.status));
     LAUNCHCHK();
   } else {
     hipLaunchKernelGGL(sky::k_bmasks<D>, dim3(gN), dim3(kBlock), 0, c->stream,

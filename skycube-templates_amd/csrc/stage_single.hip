@@ -250,7 +250,7 @@ int split_d(sky_ctx* c, const float* rows_dev, uint32_t n_rows, uint32_t d, sky:
                        dim3(kBlock), 0, c->stream, P<float>(c->s1rows), m, sminkey);
     LAUNCHCHK();
   }
-  hipLaunchKernelGGL(sky::k_self_idx<D>, dim3((m + 3) / 4), dim3(kBlock), 0, c->stream,
+  hipLaunchKernelGGL(sky::k_self_idx<D>, dim3(m), dim3(64), 0, c->stream,
                      P<float>(c->s1rows), P<uint32_t>(c->s1keys), P<uint2>(c->codes),
                      P<uint32_t>(c->boff), P<uint32_t>(c->bidx), P<uint32_t>(c->bkey),
                      P<uint2>(c->bcode), m, need_dom, need_dom ? nullptr : sminkey,
