@@ -564,7 +564,9 @@ constexpr int narrow_zbits() {
 
 // Points per workgroup of k_member: as many as keep its LDS near 64 KB,
 // and no more than 4 from d = 11 on (measured: the points of a workgroup
-// share its candidate groups; more points make more groups live).
+// share its candidate groups; more points make more groups live).  The
+// single-instance launch at d = 16 with the tie lattices takes one point
+// per workgroup instead (launch_member).
 template <int D>
 constexpr int member_points() {
   return D >= 15 ? 2 : (D >= 11 ? 4 : 8);
