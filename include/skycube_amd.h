@@ -244,7 +244,10 @@ int sky_batch_mdmc_dev(sky_ctx *ctx, const float *const *inst_rows_dev,
                        const uint32_t *max_d, uint64_t *total_ext);
 
 /* The HIP stream (a hipStream_t) that every call on the context enqueues
- * its work on, for callers that order their own work with it. */
+ * its work on, for callers that order their own work with it.  (Calls may
+ * also run work on a second stream of the context -- the d = 16
+ * membership's deferred points, the HashCube's early copy of the ids --
+ * which always rejoins this one before the call's results are ready.) */
 void *sky_ctx_stream(sky_ctx *ctx);
 
 /* Results of the last batch, in instance order: ext_off and dom_off
