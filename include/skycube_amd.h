@@ -247,7 +247,8 @@ int sky_batch_mdmc_dev(sky_ctx *ctx, const float *const *inst_rows_dev,
  * its work on, for callers that order their own work with it.  (Calls may
  * also run work on a second stream of the context -- the d = 16
  * membership's deferred points, the HashCube's early copy of the ids --
- * which always rejoins this one before the call's results are ready.) */
+ * whose work is joined to this stream, or waited for, before the call
+ * returns.) */
 void *sky_ctx_stream(sky_ctx *ctx);
 
 /* Results of the last batch, in instance order: ext_off and dom_off
